@@ -1,0 +1,399 @@
+"""Seeded scenario generators for the ADI timestep.
+
+A scenario is the full input set of one run in the reference's padded
+layout: (N+3) x (M+3) row-major arrays (rows i, columns j, unit stride j),
+hourly boundary series per side and the sediment switches.
+
+Depth convention (bandodosau): h is the bed depth below datum at cell
+corners; a cell is wet when the mean of its four corners exceeds NANGDAY
+(support_funcs.cpp:51).  Land is h = -3 (bed 3 m above datum).
+
+The generators build corner depths so that every wet cell has four
+positive corners and every partly-wet cell is dry.  Otherwise the
+reference formulas take pow() of a negative face depth (Kx1/Ky1,
+support_funcs.cpp:64-67; Q-boundary weights, :509-514) or divide by a zero
+face depth, and produce NaN on the first step.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+LAND = -3.0
+
+
+@dataclass
+class Scenario:
+    name: str
+    M: int
+    N: int
+    h: np.ndarray
+    hsnham: np.ndarray
+    visc: np.ndarray
+    Fw: np.ndarray
+    total_time: int
+    boundary_type: tuple
+    bc: list
+    CC: list = None
+    sediment_start: float = 1e30
+    bed_change_start: float = 1e30
+    coeff_overrides: dict = field(default_factory=dict)
+    u0: np.ndarray = None
+    v0: np.ndarray = None
+    z0: np.ndarray = None
+    FS0: np.ndarray = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def shape(self):
+        return (self.N + 3, self.M + 3)
+
+    @property
+    def cells(self):
+        """Mesh cells counted by the cell-steps/s metric (all cells, wet or dry)."""
+        return self.M * self.N
+
+
+def _edge_pad(core):
+    """standardize_inp.py's np.pad(..., ((1, 2), (1, 2)), 'edge')."""
+    return np.pad(core, ((1, 2), (1, 2)), mode="edge")
+
+
+def _wet_mask(h):
+    """Onetime_init's mask for padded cells (i, j), i >= 1, j >= 1."""
+    m = np.zeros(h.shape, dtype=bool)
+    s = h[:-1, :-1] + h[:-1, 1:] + h[1:, :-1] + h[1:, 1:]
+    m[1:, 1:] = s * 0.25 > 0.0
+    return m
+
+
+def build_bathymetry(M, N, in_channel_fn, depth_fn):
+    """Corner depths on the padded grid from a cell-membership function and
+    a positive corner depth profile.
+
+    A corner is positive (depth_fn) when any cell sharing it is in the
+    channel.  Each land corner is then set below minus the sum of the
+    positive corners of every cell it touches, so a cell with a land corner
+    always has a non-positive corner mean (dry), and a cell is wet exactly
+    when its four corners are positive.  Depths on faces between a land
+    corner and a positive corner are negative; such faces belong only to
+    dry cells, whose coefficients the timestep never reads."""
+    H, W = N + 3, M + 3
+    ii, jj = np.meshgrid(np.arange(H, dtype=np.float64), np.arange(W, dtype=np.float64), indexing="ij")
+    # cell (i, j) centre in core coordinates (i - 1.5, j - 1.5); corner (a, b) at (a - 1, b - 1)
+    cells = np.zeros((H, W), dtype=bool)
+    cells[1:, 1:] = in_channel_fn(ii[1:, 1:] - 1.5, jj[1:, 1:] - 1.5)
+    depth = np.maximum(depth_fn(ii - 1.0, jj - 1.0), 0.05)
+    pos = np.zeros((H, W), dtype=bool)  # corner positivity; cell (i,j) owns corners (i-1..i, j-1..j)
+    pos[:-1, :-1] |= cells[1:, 1:]
+    pos[:-1, 1:] |= cells[1:, 1:]
+    pos[1:, :-1] |= cells[1:, 1:]
+    pos[1:, 1:] |= cells[1:, 1:]
+    h = np.where(pos, depth, 0.0)
+    # per cell: sum of its positive corners
+    cs = np.zeros((H, W))
+    cs[1:, 1:] = h[:-1, :-1] + h[:-1, 1:] + h[1:, :-1] + h[1:, 1:]
+    # per corner: max over the (up to 4) cells sharing it
+    need = np.zeros((H, W))
+    need[:-1, :-1] = np.maximum(need[:-1, :-1], cs[1:, 1:])
+    need[:-1, 1:] = np.maximum(need[:-1, 1:], cs[1:, 1:])
+    need[1:, :-1] = np.maximum(need[1:, :-1], cs[1:, 1:])
+    need[1:, 1:] = np.maximum(need[1:, 1:], cs[1:, 1:])
+    land = np.minimum(LAND, -need - 0.01)
+    h = np.where(pos, depth, land)
+    h = _edge_pad(h[1:N + 1, 1:M + 1])
+    wet = _wet_mask(h)
+    allpos = np.zeros((H, W), dtype=bool)
+    allpos[1:, 1:] = (h[:-1, :-1] > 0) & (h[:-1, 1:] > 0) & (h[1:, :-1] > 0) & (h[1:, 1:] > 0)
+    assert (wet[2:N + 1, 2:M + 1] == allpos[2:N + 1, 2:M + 1]).all()
+    return np.ascontiguousarray(h)
+
+
+def _series(values, total_time):
+    v = np.asarray(values, dtype=np.float64).reshape(-1)
+    if v.size == 1:
+        v = np.full(total_time, float(v[0]))
+    assert v.size >= total_time
+    return np.ascontiguousarray(v[:total_time])
+
+
+def meander_channel(M, N, seed=1, amplitude=20.0, wavelength=200.0, half_width=20.0,
+                    depth_bank=2.0, depth_max=8.0, Q=-300.0, hours=1, manning=(0.025, 0.035),
+                    visc=5.0, Fw=0.0, tide=None, manning_factor=1.0, name=None):
+    """Config-1 family: sinusoidal channel along j with a parabolic section,
+    land outside; up side (j = M) Q inflow, down side (j = 2) water level."""
+    rng = np.random.default_rng(seed)
+    yc0 = N / 2.0
+
+    def centre(x):
+        return yc0 + amplitude * np.sin(2 * np.pi * x / wavelength)
+
+    def in_channel(y, x):
+        return np.abs(y - centre(x)) <= half_width
+
+    def depth(y, x):
+        r = np.abs(y - centre(x)) / half_width
+        return depth_bank + (depth_max - depth_bank) * np.maximum(0.0, 1.0 - r * r)
+
+    h = build_bathymetry(M, N, in_channel, depth)
+    shape = h.shape
+    hsn = rng.uniform(manning[0], manning[1], size=shape) * manning_factor
+    total_time = hours + 3
+    bc_up = _series(Q, total_time)
+    if tide is None:
+        bc_down = _series(0.0, total_time)
+    else:
+        amp, period_h = tide
+        t = np.arange(total_time, dtype=np.float64)
+        bc_down = amp * np.sin(2 * np.pi * t / period_h)
+    zeros = _series(0.0, total_time)
+    return Scenario(
+        name=name or f"meander_{M}x{N}", M=M, N=N, h=h, hsnham=hsn,
+        visc=np.full(shape, visc), Fw=np.full(shape, Fw), total_time=total_time,
+        boundary_type=(2, 0, 0, 0), bc=[bc_up, bc_down, zeros, zeros], CC=None,
+        meta={"seed": seed, "Q": Q},
+    )
+
+
+def config1(seed=1):
+    """256 x 128, dx = 10 m: amplitude 20, wavelength 200, width 40 cells,
+    depth 2-8 m, Manning U[0.025, 0.035], A = 5, Fw = 0; Q = -300 m^3/s in at
+    the up side, z = 0 at the down side; 1 simulated hour."""
+    return meander_channel(256, 128, seed=seed, name="config1_256x128")
+
+
+def meander_islands(M, N, seed=2, n_islands=30, amplitude=None, wavelength=None,
+                    half_width=None, depth_bank=1.0, depth_max=15.0, Q=-4000.0, hours=1,
+                    fs_range=(0.05, 0.2), Fw=0.0025, sediment_start=300.0, name=None):
+    """Config-2 family: wide meandering channel with elliptic islands
+    (radius 5-40 cells), depth 1-15 m tapering to 1 m at banks and island
+    shores; up side Q inflow with FS concentration U[0.05, 0.2] kg/m^3,
+    down side water level."""
+    rng = np.random.default_rng(seed)
+    amplitude = amplitude if amplitude is not None else 0.12 * N
+    wavelength = wavelength if wavelength is not None else 0.8 * M
+    half_width = half_width if half_width is not None else 0.30 * N
+    yc0 = N / 2.0
+
+    def centre(x):
+        return yc0 + amplitude * np.sin(2 * np.pi * x / wavelength)
+
+    islands = []
+    while len(islands) < n_islands:
+        a, b = rng.uniform(5, 40, size=2)
+        cx = rng.uniform(0.05 * M, 0.95 * M)
+        cy = centre(cx) + rng.uniform(-1, 1) * max(half_width - max(a, b) - 10, 1.0)
+        islands.append((cy, cx, b, a))  # (row centre, col centre, row radius, col radius)
+
+    def island_rho(y, x):
+        rho = np.full(np.broadcast(y, x).shape, np.inf)
+        for cy, cx, ry, rx in islands:
+            r = np.sqrt(((y - cy) / ry) ** 2 + ((x - cx) / rx) ** 2)
+            rho = np.minimum(rho, (r - 1.0) * min(ry, rx))
+        return rho  # distance-like: < 0 inside an island
+
+    def in_channel(y, x):
+        return (np.abs(y - centre(x)) <= half_width) & (island_rho(y, x) > 0)
+
+    def depth(y, x):
+        r = np.abs(y - centre(x)) / half_width
+        d = depth_bank + (depth_max - depth_bank) * np.maximum(0.0, 1.0 - r * r)
+        shore = depth_bank + 0.3 * np.maximum(island_rho(y, x), 0.0)
+        return np.minimum(d, shore)
+
+    h = build_bathymetry(M, N, in_channel, depth)
+    shape = h.shape
+    hsn = rng.uniform(0.025, 0.035, size=shape)
+    total_time = hours + 3
+    zeros = _series(0.0, total_time)
+    cc_up = rng.uniform(fs_range[0], fs_range[1], size=total_time)
+    return Scenario(
+        name=name or f"islands_{M}x{N}", M=M, N=N, h=h, hsnham=hsn,
+        visc=np.full(shape, 5.0), Fw=np.full(shape, Fw), total_time=total_time,
+        boundary_type=(2, 0, 0, 0), bc=[_series(Q, total_time), zeros, zeros, zeros],
+        CC=[cc_up, zeros, zeros, zeros], sediment_start=sediment_start,
+        bed_change_start=sediment_start,
+        coeff_overrides={"dm": 2e-4, "Toe": 0.1, "Tod": 0.06},
+        meta={"seed": seed, "islands": len(islands), "Q": Q},
+    )
+
+
+def config2(seed=2):
+    """2048 x 1024 meander + 30 islands, sediment from t = 300 s, 1 hour."""
+    return meander_islands(2048, 1024, seed=seed, name="config2_2048x1024")
+
+
+def open_box(M, N, seed=3, n_islands=3, depth=(3.0, 6.0), Q_up=-150.0, Q_left=-80.0,
+             z_down=0.05, z_right=0.0, types=(2, 0, 2, 0), hours=1, fs=True, name=None):
+    """Small fully-open basin with islands: every side carries a boundary, so
+    both sweeps see Q starts/ends, water-level starts/ends and wall ends."""
+    rng = np.random.default_rng(seed)
+    islands = []
+    for _ in range(n_islands):
+        cy = rng.uniform(0.25 * N, 0.75 * N)
+        cx = rng.uniform(0.25 * M, 0.75 * M)
+        ry, rx = rng.uniform(1.5, max(2.0, 0.12 * N)), rng.uniform(1.5, max(2.0, 0.12 * M))
+        islands.append((cy, cx, ry, rx))
+
+    def island_rho(y, x):
+        rho = np.full(np.broadcast(y, x).shape, np.inf)
+        for cy, cx, ry, rx in islands:
+            r = np.sqrt(((y - cy) / ry) ** 2 + ((x - cx) / rx) ** 2)
+            rho = np.minimum(rho, (r - 1.0) * min(ry, rx))
+        return rho
+
+    base = rng.uniform(depth[0], depth[1])
+
+    def in_channel(y, x):
+        return island_rho(y, x) > 0
+
+    def dep(y, x):
+        d = base + (depth[1] - depth[0]) * 0.5 * (1 + np.sin(0.3 * x) * np.cos(0.2 * y))
+        shore = 1.0 + 0.5 * np.maximum(island_rho(y, x), 0.0)
+        return np.minimum(d, shore)
+
+    h = build_bathymetry(M, N, in_channel, dep)
+    shape = h.shape
+    total_time = hours + 3
+    t = np.arange(total_time, dtype=np.float64)
+    bcs = [
+        _series(Q_up, total_time) * (1 + 0.1 * np.sin(t)),
+        z_down * np.cos(0.7 * t),
+        _series(Q_left, total_time),
+        z_right + 0.02 * np.sin(0.5 * t),
+    ]
+    bcs = [np.ascontiguousarray(b, dtype=np.float64) for b in bcs]
+    cc = [rng.uniform(0.05, 0.2, size=total_time) for _ in range(4)] if fs else None
+    return Scenario(
+        name=name or f"openbox_{M}x{N}", M=M, N=N, h=h,
+        hsnham=rng.uniform(0.02, 0.04, size=shape), visc=np.full(shape, 5.0),
+        Fw=np.full(shape, 0.003), total_time=total_time, boundary_type=tuple(types),
+        bc=bcs, CC=cc, sediment_start=0.0, bed_change_start=0.0,
+        coeff_overrides={"dm": 2e-4, "Toe": 0.1, "Tod": 0.06},
+        meta={"seed": seed},
+    )
+
+
+def closed_basin(M, N, seed=4, depth=4.0, z0_amp=0.0, name=None):
+    """Basin with land on every side (no boundary segments): with z0 = 0 it
+    is a lake at rest; with a bump in z0 it sloshes and conserves volume."""
+    rng = np.random.default_rng(seed)
+
+    def in_channel(y, x):
+        return (y > 2) & (y < N - 4) & (x > 2) & (x < M - 4)
+
+    def dep(y, x):
+        return np.full(np.broadcast(y, x).shape, depth)
+
+    h = build_bathymetry(M, N, in_channel, dep)
+    shape = h.shape
+    total_time = 4
+    zeros = _series(0.0, total_time)
+    z0 = None
+    if z0_amp:
+        ii, jj = np.meshgrid(np.arange(N + 3), np.arange(M + 3), indexing="ij")
+        z0 = z0_amp * np.exp(-(((ii - N / 2) / (N / 6)) ** 2 + ((jj - M / 2) / (M / 6)) ** 2))
+        z0 = np.ascontiguousarray(z0)
+    return Scenario(
+        name=name or f"basin_{M}x{N}", M=M, N=N, h=h,
+        hsnham=rng.uniform(0.025, 0.035, size=shape), visc=np.full(shape, 5.0),
+        Fw=np.zeros(shape), total_time=total_time, boundary_type=(0, 0, 0, 0),
+        bc=[zeros, zeros, zeros, zeros], CC=None, z0=z0, meta={"seed": seed},
+    )
+
+
+def braided(M, N, seed=5, n_channels=12, width=(10, 120), depth=(1.0, 10.0), Q=-2000.0,
+            hours=1, name=None):
+    """Config-3 family: braided river of random-walk channels running along
+    j with widths drawn from `width` (cells)."""
+    rng = np.random.default_rng(seed)
+    paths = []
+    xs = np.arange(M + 3, dtype=np.float64)
+    for _ in range(n_channels):
+        y = rng.uniform(0.15 * N, 0.85 * N)
+        steps = rng.normal(0, 1.5, size=M + 3)
+        yc = np.clip(y + np.cumsum(steps), 0.1 * N, 0.9 * N)
+        # smooth the walk
+        k = np.ones(31) / 31.0
+        yc = np.convolve(np.pad(yc, 15, mode="edge"), k, mode="valid")
+        w = rng.uniform(width[0], width[1]) / 2.0
+        paths.append((yc, w))
+
+    def dist(y, x):
+        xi = np.clip(np.rint(x).astype(np.int64), 0, M + 2)
+        d = np.full(np.broadcast(y, x).shape, np.inf)
+        for yc, w in paths:
+            d = np.minimum(d, np.abs(y - yc[xi]) / w)
+        return d  # <= 1 inside some channel
+
+    def in_channel(y, x):
+        return dist(y, x) <= 1.0
+
+    def dep(y, x):
+        r = dist(y, x)
+        return depth[0] + (depth[1] - depth[0]) * np.maximum(0.0, 1.0 - r * r)
+
+    h = build_bathymetry(M, N, in_channel, dep)
+    shape = h.shape
+    total_time = hours + 3
+    zeros = _series(0.0, total_time)
+    return Scenario(
+        name=name or f"braided_{M}x{N}", M=M, N=N, h=h,
+        hsnham=rng.uniform(0.025, 0.035, size=shape), visc=np.full(shape, 5.0),
+        Fw=np.full(shape, 0.0025), total_time=total_time, boundary_type=(2, 0, 0, 0),
+        bc=[_series(Q, total_time), zeros, zeros, zeros], CC=None, meta={"seed": seed},
+    )
+
+
+def config3(seed=5):
+    return braided(4096, 4096, seed=seed, name="config3_4096x4096")
+
+
+def ensemble_member(k, M=512, N=256):
+    """Config-4 member k: the config-1 generator at 512 x 256 with seed k,
+    inflow amplitude U[200, 1500] m^3/s, tide amplitude U[0.1, 0.8] m and a
+    Manning factor U[0.8, 1.2]."""
+    rng = np.random.default_rng(10_000 + k)
+    Q = -rng.uniform(200, 1500)
+    tide = rng.uniform(0.1, 0.8)
+    mf = rng.uniform(0.8, 1.2)
+    s = meander_channel(M, N, seed=k, amplitude=40.0, wavelength=400.0, half_width=40.0,
+                        Q=Q, tide=(tide, 12.42), manning_factor=mf, name=f"member{k}_{M}x{N}")
+    s.meta.update({"tide": tide, "manning_factor": mf})
+    return s
+
+
+def segment_histogram(h, M, N):
+    """Wet-segment lengths along rows and columns (reference segmentation
+    rules) from a corner-depth field, for generator reports."""
+    wet = _wet_mask(h)
+    kh = np.where(wet, 0, 2)
+    kh[0, :] = 2
+    kh[:, 0] = 2
+    lens = []
+    for line, end in ((kh[i, :], M) for i in range(2, N + 1)):
+        lens.extend(_runs(line, end))
+    for line, end in ((kh[:, j], N) for j in range(2, M + 1)):
+        lens.extend(_runs(line, end))
+    return np.asarray(lens)
+
+
+def _runs(line, end):
+    out = []
+    start = 2
+    while start < end:
+        if line[start] != 0:
+            while line[start] and start < end:
+                start += 1
+        if start + 1 == end:
+            start = end
+        if line[start] == 0 and start + 1 < end:
+            e = start
+            while line[e] == 0 and e < end:
+                e += 1
+            if line[e] != 0 and e <= end:
+                out.append(e - 1 - start + 1)
+                start = e
+            else:
+                out.append(end - start + 1)
+                start = end
+    return out

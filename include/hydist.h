@@ -1,0 +1,221 @@
+/*
+ * hydist.h - C ABI of the MI355X (gfx950) implementation of the Hydist_C
+ * ADI shallow-water + suspended-sediment timestep.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * an int status (0 = OK, negative = HD_E_*) and never throws.  A context
+ * (hd_ctx) owns every device buffer of one scenario and one HIP stream;
+ * calls on a context are asynchronous until hd_sync() unless stated.
+ *
+ * Field layout (the reference's, loader.cpp:400, support_funcs.cpp:39-43):
+ * 2-D fields are row-major [i][j], i in [0, N+2] (rows), j in [0, M+2]
+ * (columns, unit stride).  Compute cells are i in [2, N], j in [2, M].
+ * Host buffers passed to the API are dense (N+3) x (M+3); the device copy
+ * uses a padded pitch chosen by the library.
+ *
+ * Each declaration names the reference interface it replaces (file:line,
+ * in the Hydist_C sources).
+ */
+#ifndef HYDIST_H
+#define HYDIST_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define HD_OK 0
+#define HD_E_INVALID (-1)   /* bad argument / shape */
+#define HD_E_NOMEM (-2)     /* device allocation failed / over budget */
+#define HD_E_HIP (-3)       /* HIP runtime error (message in hd_last_error) */
+#define HD_E_NUMERIC (-4)   /* NaN/Inf watchdog tripped */
+#define HD_E_STATE (-5)     /* call out of order (e.g. step before init) */
+#define HD_E_NODEV (-6)     /* no usable gfx950 device */
+#define HD_E_FAULT (-7)     /* injected fault (test hook) */
+#define HD_E_CANCELLED (-8) /* serving request cancelled */
+#define HD_E_UNSUPPORTED (-9)
+
+/* ------------------------------------------------------------------ */
+/* Constant_Coeffs: byte-for-byte the reference struct (engine.h:39-52) */
+/* ------------------------------------------------------------------ */
+typedef struct hd_coeffs {
+    double dX, dY, dT, dXbp, dYbp, dX2, dY2, dTchia2dX, dTchia2dY, HaiChiadT;
+    double QuyDoiTime, QuyDoiPi;
+    double NANGDAY, H_TINH, heso, mu_mn;
+    double kenhhepng, kenhhepd;
+    double Wind, huonggio, Zbandau;
+    double NDnen, NDbphai, NDbtrai, NDbtren, NDbduoi;
+    double Tod, Toe, hstoe, ghtoe, Mbochat;
+    double ro, ros, dm, muy, d90, Dorong, Sx, sohn, soha, sohn1;
+    double CORIOLIS_FORCE, g, Ks;
+    double Windx, Windy, Ufr, Dxr, wss;
+} hd_coeffs;
+
+/* ------------------------------------------------------------------ */
+/* Options: the data members of the reference struct (engine.h:16-37)  */
+/* in the same order and types (C bool == C++ bool on x86-64).         */
+/* ------------------------------------------------------------------ */
+typedef struct hd_options {
+    double Tmax, t_start, sediment_start, bed_change_start;
+    int interval;
+    bool kenhhepd, kenhhepng;
+    int M, N;
+    bool ketdinh, channel, debug, plot;
+    int total_time;
+} hd_options;
+
+/* Boundary sides, in the order Update_Boundary_Value visits them
+ * (support_funcs.cpp:692-725).  boundary_type[side] == 2 means a discharge
+ * (Q) boundary, anything else a water-level (Z) boundary
+ * (loader.cpp:158-164). */
+enum { HD_SIDE_UP = 0, HD_SIDE_DOWN = 1, HD_SIDE_LEFT = 2, HD_SIDE_RIGHT = 3 };
+
+/* ------------------------------------------------------------------ */
+/* Mesh + inputs of one scenario (replaces load_inputs' 13 vectors,    */
+/* loader.cpp:120-168, and Host_arrays, engine.h:80-88).               */
+/* ------------------------------------------------------------------ */
+typedef struct hd_mesh {
+    int M, N;                        /* arrays are (N+3) x (M+3), row-major */
+    const double *h;                 /* bandodosau: depth at cell corners (required) */
+    const double *hsnham;            /* Manning-like roughness (required) */
+    const double *VISCOIDX;          /* horizontal eddy viscosity A (required) */
+    const double *Fw;                /* bed friction factor (required when sediment is on; NULL = 0) */
+    const double *Tsxw, *Tsyw;       /* wind stress fields (NULL = 0) */
+    const double *u0, *v0, *z0, *FS0; /* initial state (NULL = 0) */
+    int total_time;                  /* samples (hourly) per boundary series, >= hours run + 2 */
+    int boundary_type[4];            /* per side, see HD_SIDE_* */
+    const double *bc[4];             /* per side: total_time hourly samples (Z in m or Q in m^3/s) */
+    const double *CC[4];             /* per side: total_time hourly FS samples (kg/m^3); NULL = 0 */
+} hd_mesh;
+
+/* Numerical forms of the implicit sweeps. */
+enum {
+    HD_FORM_FP64_PLAIN = 0,  /* interleaved (z,u) Thomas in the reference's order: bitwise FP64 */
+    HD_FORM_FP64_FAST = 1,   /* z-only (Schur) system, partitioned parallel solve, FP64 */
+    HD_FORM_FP32_REFINE = 2  /* z-only system solved in FP32 + one FP64 residual refinement */
+};
+
+/* Semantics switches for reference behaviours that are defined differently
+ * by default (each documented where it is implemented). */
+#define HD_QUIRK_Q11_LITERAL 0x1 /* open-start segments test the start side's Q flag at their end,
+                                    exactly as UVZSolver_multithread.cpp:438 */
+
+typedef struct hd_config {
+    int form;          /* HD_FORM_* */
+    int sediment;      /* run the FS sweeps once t >= sediment_start (engine.cpp:149,258) */
+    int bed_change;    /* accumulate dH (Calculate_Qb + BedLoad) once t >= bed_change_start */
+    int quirks;        /* HD_QUIRK_* bits */
+    int device;        /* HIP device ordinal */
+} hd_config;
+
+/* Field ids for hd_copy_out / hd_copy_in. */
+enum {
+    HD_F_U = 0, HD_F_V, HD_F_Z, HD_F_TU, HD_F_TV, HD_F_TZ,
+    HD_F_HMOI, HD_F_HTDU, HD_F_HTDV, HD_F_FS, HD_F_TFS,
+    HD_F_VTH, HD_F_KX, HD_F_KY, HD_F_QBX, HD_F_QBY, HD_F_DH,
+    HD_F_HTAIZ, HD_F_HTAIZ_BD, HD_F_KX1, HD_F_KY1,
+    HD_F_H, HD_F_HSNHAM, HD_F_VISCOIDX, HD_F_FW,
+    HD_F_KHOUOT, /* int32 field: copy with hd_copy_out_i32 / hd_copy_in_i32 */
+    HD_F_COUNT
+};
+
+/* Boundary work arrays (device; Argument_Pointers ubt/ubp/vbt/vbd/hi,
+ * engine.h:61-65, sizes from loader.cpp:306-320). */
+enum { HD_B_UBT = 0, HD_B_UBP, HD_B_VBT, HD_B_VBD, HD_B_HI, HD_B_COUNT };
+
+typedef struct hd_ctx hd_ctx;
+
+/* Load_coeffs (loader.cpp:4-87): fills the reference's constants. */
+int hd_load_coeffs(hd_coeffs *c);
+
+/* Recomputes the members Load_coeffs derives (dXbp, dX2, dTchia2dX,
+ * HaiChiadT, Ks, Windx/Windy, Ufr, Dxr, wss; loader.cpp:9-17,79-86) from
+ * the primary ones, for callers that change dX, dT, dm, ... */
+int hd_finalize_coeffs(hd_coeffs *c);
+
+/* attribute_arrays_memory_alloc + supporting_arrays_alloc
+ * (loader.cpp:224-375, main.cpp:216-233): allocate every device buffer of
+ * one scenario and upload its inputs. */
+int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops,
+              const hd_config *cfg, hd_ctx **out);
+
+/* main.cpp:257-299: Onetime_init, Find_Calculation_limits_{Horizontal,
+ * Vertical}, Htuongdoi and preprocess_data, on the device. */
+int hd_init(hd_ctx *ctx);
+
+/* update_boundary_at_t (engine.cpp:22-36) -> Update_Boundary_Value. */
+int hd_update_boundary(hd_ctx *ctx, double t);
+
+/* First half-step (engine.cpp:66-170): t is the time after t += dT/2.
+ * Runs the boundary update, the UZ implicit sweep, explicit V, smoothing,
+ * wet-depth update, segmentation, Htuongdoi and, when enabled and
+ * t >= sediment_start, the FS sweep along i. */
+int hd_halfstep_uz(hd_ctx *ctx, double t);
+
+/* Second half-step (engine.cpp:176-288), including Reset_states_vertical,
+ * the FS sweep along j and (when enabled) the bed change. */
+int hd_halfstep_vz(hd_ctx *ctx, double t);
+
+/* One full dT from *t (both half-steps); advances *t by dT. */
+int hd_step(hd_ctx *ctx, double *t);
+
+/* Hydraulic_Calculation's loop (engine.cpp:61-295): steps while t < Tmax
+ * (Tmax truncated to int as engine.cpp:62).  *t holds the start time on
+ * entry and the final time on return. */
+int hd_run(hd_ctx *ctx, double *t, double Tmax);
+
+/* Block until all work queued on the context has finished. */
+int hd_sync(hd_ctx *ctx);
+
+/* save_result / save_FS (loader.cpp:397-434) data access: dense
+ * (N+3) x (M+3) copies between host and the device field. n is the
+ * element count of the host buffer. */
+int hd_copy_out(hd_ctx *ctx, int field, double *dst, size_t n);
+int hd_copy_in(hd_ctx *ctx, int field, const double *src, size_t n);
+int hd_copy_out_i32(hd_ctx *ctx, int field, int32_t *dst, size_t n);
+int hd_copy_in_i32(hd_ctx *ctx, int field, const int32_t *src, size_t n);
+
+/* Boundary work arrays: HD_B_UBT/UBP have M+3 entries, VBT/VBD N+3,
+ * HD_B_HI 2*(M+N+6) (layout of preprocess_data, support_funcs.cpp:505-570). */
+int hd_copy_out_boundary(hd_ctx *ctx, int which, double *dst, size_t n);
+int hd_copy_in_boundary(hd_ctx *ctx, int which, const double *src, size_t n);
+/* hmax_u, hmax_d, hmax_l, hmax_r (Argument_Pointers, engine.h:57). */
+int hd_get_hmax(hd_ctx *ctx, double out[4]);
+int hd_set_hmax(hd_ctx *ctx, const double in[4]);
+
+/* Wet-segment tables (daui/cuoii/moci and dauj/cuoij/mocj,
+ * support_funcs.cpp:238-343) as CSR: dir 0 = along j per row i (lines
+ * i = 0..N+2), dir 1 = along i per column j (lines j = 0..M+2).
+ * counts[line] receives the number of segments, first/last the segment
+ * bounds in line order; cap is the capacity of first/last.  *total
+ * receives the total segment count (may exceed cap: then only cap are
+ * written). */
+int hd_get_segments(hd_ctx *ctx, int dir, int32_t *counts, size_t ncounts,
+                    int32_t *first, int32_t *last, size_t cap, int64_t *total);
+
+/* Domain water volume sum over wet compute cells of (htaiz + z)*dX*dY and
+ * the running sum of boundary inflow volume, both accumulated in FP64 on
+ * the device in a fixed order. */
+int hd_mass_balance(hd_ctx *ctx, double *volume, double *inflow);
+
+/* Frees every buffer of the context. */
+int hd_destroy(hd_ctx *ctx);
+
+/* Last error message of the context (or of the library when ctx is NULL). */
+const char *hd_last_error(hd_ctx *ctx);
+
+/* Library/device information: writes a short text (arch, CU count,
+ * build flags) into buf. */
+int hd_device_info(char *buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDIST_H */

@@ -1,0 +1,79 @@
+/*
+ * hydist_oracle.h - CPU FP64 oracle for the Hydist_C ADI timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU implementation is
+ * compared against; it is never linked into or called by the product
+ * library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline may load it.
+ *
+ * The API mirrors include/hydist.h (same mesh/coeff/option structs, same
+ * field ids) so a test can drive both with identical inputs.
+ */
+#ifndef HYDIST_ORACLE_H
+#define HYDIST_ORACLE_H
+
+#include "hydist.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ho_ctx ho_ctx;
+
+/* Load_coeffs (loader.cpp:4-87), restated. */
+int ho_load_coeffs(hd_coeffs *c);
+/* Recompute Load_coeffs' derived members (dX2, Ks, Ufr, Dxr, wss, ...)
+ * from the primary ones. */
+int ho_finalize_coeffs(hd_coeffs *c);
+
+int ho_set_threads(int n);
+int ho_get_threads(void);
+
+int ho_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops,
+              const hd_config *cfg, ho_ctx **out);
+int ho_init(ho_ctx *o);
+int ho_update_boundary(ho_ctx *o, double t);
+int ho_halfstep_uz(ho_ctx *o, double t);
+int ho_halfstep_vz(ho_ctx *o, double t);
+int ho_step(ho_ctx *o, double *t);
+int ho_run(ho_ctx *o, double *t, double Tmax);
+int ho_copy_out(ho_ctx *o, int field, double *dst, size_t n);
+int ho_copy_in(ho_ctx *o, int field, const double *src, size_t n);
+int ho_copy_out_i32(ho_ctx *o, int field, int32_t *dst, size_t n);
+int ho_copy_in_i32(ho_ctx *o, int field, const int32_t *src, size_t n);
+int ho_copy_out_boundary(ho_ctx *o, int which, double *dst, size_t n);
+int ho_copy_in_boundary(ho_ctx *o, int which, const double *src, size_t n);
+int ho_get_hmax(ho_ctx *o, double out[4]);
+int ho_set_hmax(ho_ctx *o, const double in[4]);
+int ho_get_segments(ho_ctx *o, int dir, int32_t *counts, size_t ncounts, int32_t *first,
+                    int32_t *last, size_t cap, int64_t *total);
+int ho_mass_balance(ho_ctx *o, double *volume, double *inflow);
+int ho_destroy(ho_ctx *o);
+const char *ho_last_error(ho_ctx *o);
+
+/* Stage-level pieces, exposed for unit tests. */
+
+/* tridiag (UVZSolver_multithread.cpp:19-43): Thomas without pivoting on
+ * rows 0..sn. Ap/Bp are caller scratch of sn+1 entries. */
+int ho_thomas(int sn, const double *AA, const double *BB, const double *CC, const double *DD,
+              double *x, double *Ap, double *Bp);
+
+/* Run-length wet-segment rule of Find_Calculation_limits_Horizontal
+ * (support_funcs.cpp:260-288) on one line: wet[k] for k in [0, end+1]
+ * holds khouot values along the line, end is M (or N).  Writes up to cap
+ * segments and returns the count. */
+int ho_segment_line(const int32_t *line, int end, int32_t *first, int32_t *last, int cap);
+
+/* Assemble the interleaved (z,u) system of one hydraulic segment and
+ * solve it (stage A4..A7 on a single line) - exposed so tests can
+ * check the assembled system against its solution. dir 1 = UZ (column
+ * `line`), dir 0 = VZ (row `line`).  Writes sn+1 rows into AA..DD and x
+ * (capacity cap) and returns sn (or a negative status). */
+int ho_line_system(ho_ctx *o, int dir, int line, int seg, double *AA, double *BB, double *CC,
+                   double *DD, double *x, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
