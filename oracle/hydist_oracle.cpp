@@ -1794,6 +1794,15 @@ int ho_thomas(int sn, const double *AA, const double *BB, const double *CC, cons
     return HD_OK;
 }
 
+double ho_source_chs(const hd_coeffs *c, double wsm, double Zf, double fs, double fw, double vth,
+                     double h_moi) {
+    return source_chs(*c, wsm, Zf, fs, fw, vth, h_moi);
+}
+
+double ho_source_nchs(const hd_coeffs *c, double wsm, double Zf, double Uf, double h_moi, double fs) {
+    return source_nchs(*c, wsm, Zf, c->Dxr, c->Ufr, Uf, h_moi, fs);
+}
+
 int ho_segment_line(const int32_t *line, int end, int32_t *first, int32_t *last, int cap) {
     if (!line || end < 2) return HD_E_INVALID;
     std::vector<Seg> segs;

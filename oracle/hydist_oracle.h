@@ -72,6 +72,11 @@ int ho_segment_line(const int32_t *line, int end, int32_t *first, int32_t *last,
 int ho_line_system(ho_ctx *o, int dir, int line, int seg, double *AA, double *BB, double *CC,
                    double *DD, double *x, int cap);
 
+/* source_chs / source_nchs (sediment_transport.cpp:11-49). */
+double ho_source_chs(const hd_coeffs *c, double wsm, double Zf, double fs, double fw, double vth,
+                     double h_moi);
+double ho_source_nchs(const hd_coeffs *c, double wsm, double Zf, double Uf, double h_moi, double fs);
+
 #ifdef __cplusplus
 }
 #endif

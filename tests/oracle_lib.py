@@ -200,3 +200,17 @@ def segment_line(line, end):
     n = lib().ho_segment_line(line.ctypes.data_as(C.POINTER(C.c_int32)), end,
                               f.ctypes.data_as(C.POINTER(C.c_int32)), l.ctypes.data_as(C.POINTER(C.c_int32)), cap)
     return list(zip(f[:n].tolist(), l[:n].tolist()))
+
+
+def source_chs(coeffs, wsm, Zf, fs, fw, vth, h):
+    L = lib()
+    L.ho_source_chs.argtypes = [C.POINTER(abi.Coeffs)] + [C.c_double] * 6
+    L.ho_source_chs.restype = C.c_double
+    return L.ho_source_chs(C.byref(coeffs), wsm, Zf, fs, fw, vth, h)
+
+
+def source_nchs(coeffs, wsm, Zf, Uf, h, fs):
+    L = lib()
+    L.ho_source_nchs.argtypes = [C.POINTER(abi.Coeffs)] + [C.c_double] * 5
+    L.ho_source_nchs.restype = C.c_double
+    return L.ho_source_nchs(C.byref(coeffs), wsm, Zf, Uf, h, fs)
