@@ -1,0 +1,202 @@
+"""Benchmark: ADI cell-steps/s of the Hydist_C timestep on one MI355X.
+
+Workload: the 2048 x 1024 meander + 30 islands mesh with suspended
+sediment and FP64 bed change on (both from t = 300 s), dt = 2 s.  A step is
+one full dT (both ADI half-steps, engine.cpp:66,176); cell-steps/s counts
+all M*N mesh cells.  State stays resident in HBM; the timed region starts
+after the sediment switch-on so every timed step runs the full path.
+
+Prints one JSON line (see the keys below).  `roofline` is measured for the
+dominant kernel from HIP events on the library's stream; `cpu_baseline`
+times the CPU oracle on this host's cores on a bounded sample of the same
+workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ADI cell-steps/s at 2048x1024, FP64 vs FP32+refine; p50/p99 scenario latency"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per mesh cell for one full step (both half-steps),
+# FP64 state: hydraulics 204 B, + suspended sediment 48 B, + bed change 32 B.
+STEP_BYTES = {"hydraulics": 204.0, "sediment": 252.0, "bed_change": 284.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sweep_unknowns(sim, direction):
+    """Interleaved (z, velocity) unknowns of one sweep: 2L - 1 per wet
+    segment of L cells (exact for wall-ended segments, +-1 otherwise)."""
+    counts, first, last = sim.segments(direction)
+    L = (last - first + 1).astype("int64")
+    return int((2 * L - 1).sum())
+
+
+def stage_bytes(name, sim, sc):
+    """Algorithmic bytes of one launch of a stage (FP64)."""
+    cells = (sc.M + 3) * (sc.N + 3)
+    if name.startswith("k_sweep_plain"):
+        d = 1 if "uz" in name else 0
+        return 40.0 * sweep_unknowns(sim, d)  # (a, b, c, d in + x out) * 8 B per unknown
+    per_cell = {
+        "k_smooth": 8 + 4 + 8,        # t_vel + khouot in, tmp out
+        "k_solve_v": 10 * 8 + 4 + 16,  # stencil fields in, t_v + t_u out
+        "k_solve_u": 10 * 8 + 4 + 16,
+        "k_post_a": 6 * 8 + 4 + 6 * 8,
+        "k_post_b": 5 * 8 + 5 * 8,
+        "k_pre_reset": 3 * 8 + 4 + 16,
+        "k_dry": 8 + 4,
+        "k_vth_k": 9 * 8 + 3 * 8,
+        "k_update_fs": 16,
+        "k_qb": 7 * 8 + 4 + 16,
+        "k_bedload": 12 * 8 + 4 + 16,
+    }
+    for k, v in per_cell.items():
+        if name.startswith(k):
+            return float(v) * cells
+    return None
+
+
+def time_form(sc, form, steps, warmup, profile_steps):
+    from hydist_c_amd import engine
+    sim = engine.Simulation(sc, form=form, sediment=True, bed_change=True, Tmax=3600.0)
+    sim.init()
+    # advance to the sediment switch-on, then the warm-up steps
+    while sim.t < sc.sediment_start:
+        sim.step()
+    sim.step(warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / steps
+    # stage profile (separate pass: events add a little host work per launch)
+    engine.set_profiling(sim, True)
+    sim.step(profile_steps)
+    sim.sync()
+    stats = engine.stage_stats(sim)
+    engine.set_profiling(sim, False)
+    z = sim.get("Z")
+    finite = bool((z == z).all())
+    return sim, ms_per_step, stats, finite
+
+
+def cpu_baseline(sc, seconds):
+    """The oracle (OpenMP over lines) on this host, on the same mesh with
+    every stage active (sediment and bed change from t = 0), for a bounded
+    number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from hydist_c_amd.abi import make_config
+    import copy
+    s2 = copy.copy(sc)
+    s2.sediment_start = 0.0
+    s2.bed_change_start = 0.0
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    ol.set_threads(threads)
+    o = ol.Oracle(s2, cfg=make_config(sediment=True, bed_change=True))
+    o.init()
+    o.step(1)  # first touch
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    n = max(1, min(200, int(seconds / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    return {"value": sc.M * sc.N * n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} full steps (hydraulics + sediment + bed change) of the {sc.M}x{sc.N} mesh, "
+                      f"{dt:.1f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--forms", default="plain", help="comma list of plain,fast,fp32")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has a single MI355X, and this build "
+              "has no multi-GPU path (see DESIGN.md for how the mesh would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    from hydist_c_amd import abi, scenarios as S
+    forms = {"plain": abi.HD_FORM_FP64_PLAIN, "fast": abi.HD_FORM_FP64_FAST, "fp32": abi.HD_FORM_FP32_REFINE}
+    names = [f for f in args.forms.split(",") if f]
+    log("[bench] generating the 2048x1024 meander + islands mesh")
+    sc = S.config2()
+    results = {}
+    best = None
+    for name in names:
+        log(f"[bench] form {name}: warm-up to t={sc.sediment_start:.0f}s + {args.warmup} steps, "
+            f"then {args.steps} timed steps")
+        sim, ms, stats, finite = time_form(sc, forms[name], args.steps, args.warmup, args.profile_steps)
+        cs = sc.M * sc.N / (ms * 1e-3)
+        results[name] = {"cell_steps_per_s": cs, "ms_per_step": ms, "finite": finite, "stats": stats, "sim": sim}
+        log(f"[bench] form {name}: {ms:.3f} ms/step, {cs:.3e} cell-steps/s")
+        if best is None or cs > results[best]["cell_steps_per_s"]:
+            best = name
+    r = results[best]
+    sim = r["sim"]
+    stats = r["stats"]
+    # dominant kernel by total time
+    dom = max(stats.items(), key=lambda kv: kv[1][1])
+    dom_name, (launches, total_ms) = dom
+    avg_ms = total_ms / launches
+    nbytes = stage_bytes(dom_name, sim, sc)
+    roofline = None
+    if nbytes is not None:
+        achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
+    step_bytes = STEP_BYTES["bed_change"] * sc.M * sc.N
+    step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9}
+    step_roof["frac"] = step_roof["achieved_GBs"] / HBM_PEAK_GBS
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle baseline")
+        cpu = cpu_baseline(sc, args.cpu_seconds)
+        log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s ({cpu['sample']})")
+    out = {
+        "metric": METRIC,
+        "value": r["cell_steps_per_s"],
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best in ("plain", "fast") else "f32+f64",
+        "data": "synthetic (seeded generator: meander + 30 elliptic islands)",
+        "config": {"workload": "2048x1024 meander+islands, sediment + FP64 bed change, dt=2s",
+                   "M": sc.M, "N": sc.N, "form": best, "timed_from_t": sc.sediment_start},
+        "forms": {k: {"cell_steps_per_s": v["cell_steps_per_s"], "ms_per_step": v["ms_per_step"],
+                      "finite": v["finite"]} for k, v in results.items()},
+        "roofline": roofline,
+        "step_roofline": step_roof,
+        "cpu_baseline": cpu,
+        "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / v[0]} for k, v in stats.items()),
+                              key=lambda d: -d["avg_ms"] * 1)[:8],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,593 @@
+// hd_api.hip - C ABI of the gfx950 Hydist_C timestep (include/hydist.h).
+//
+// A context owns one scenario's device buffers, its descriptor (DevState,
+// kept both on the host and in device memory) and one HIP stream.  The time
+// loop of Hydraulic_Calculation (engine.cpp:61-295) is a fixed sequence of
+// kernel launches on that stream; nothing is synchronised until hd_sync or
+// a copy-out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hd_internal.h"
+
+using namespace hd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(hd_ctx *ctx, int code, const std::string &msg) {
+    if (ctx)
+        ctx->err = msg;
+    else
+        g_err = msg;
+    return code;
+}
+
+#define HD_HIP(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return set_err((ctx), HD_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+namespace hd {
+
+int ctx_alloc(hd_ctx *ctx, void **p, size_t bytes) {
+    void *q = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return set_err(ctx, HD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, bytes);
+    if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    ctx->allocs.push_back(q);
+    ctx->bytes += bytes;
+    *p = q;
+    return HD_OK;
+}
+
+double *field_ptr(hd_ctx *ctx, int f) {
+    DevState &s = ctx->hs;
+    switch (f) {
+        case HD_F_U: return s.u;
+        case HD_F_V: return s.v;
+        case HD_F_Z: return s.z;
+        case HD_F_TU: return s.t_u;
+        case HD_F_TV: return s.t_v;
+        case HD_F_TZ: return s.t_z;
+        case HD_F_HMOI: return s.H_moi;
+        case HD_F_HTDU: return s.Htdu;
+        case HD_F_HTDV: return s.Htdv;
+        case HD_F_FS: return s.FS;
+        case HD_F_TFS: return s.tFS;
+        case HD_F_VTH: return s.VTH;
+        case HD_F_KX: return s.Kx;
+        case HD_F_KY: return s.Ky;
+        case HD_F_QBX: return s.Qbx;
+        case HD_F_QBY: return s.Qby;
+        case HD_F_DH: return s.dH;
+        case HD_F_HTAIZ: return s.htaiz;
+        case HD_F_HTAIZ_BD: return s.htaiz_bd;
+        case HD_F_KX1: return s.Kx1;
+        case HD_F_KY1: return s.Ky1;
+        case HD_F_H: return const_cast<double *>(s.h);
+        case HD_F_HSNHAM: return const_cast<double *>(s.hsnham);
+        case HD_F_VISCOIDX: return const_cast<double *>(s.visc);
+        case HD_F_FW: return const_cast<double *>(s.Fw);
+        default: return nullptr;
+    }
+}
+
+Launch2D grid2d(const DevState &s, int nscen) {
+    Launch2D l;
+    l.block = dim3(64, 4, 1);
+    l.grid = dim3((s.M + 3 + 63) / 64, (s.N + 4 + 3) / 4, nscen);
+    return l;
+}
+
+// Pushes the host descriptor to the device copy.
+int sync_desc(hd_ctx *ctx) {
+    HD_HIP(ctx, hipMemcpyAsync(ctx->ds, &ctx->hs, sizeof(DevState), hipMemcpyHostToDevice, ctx->stream));
+    return HD_OK;
+}
+
+int check_launch(hd_ctx *ctx, const char *what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return HD_OK;
+}
+
+static hipEvent_t pool_get(hd_ctx *ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+hipEvent_t prof_begin(hd_ctx *ctx) {
+    if (!ctx->prof) return nullptr;
+    hipEvent_t e = pool_get(ctx);
+    if (e) (void)hipEventRecord(e, ctx->stream);
+    return e;
+}
+
+void prof_end(hd_ctx *ctx, const char *what, hipEvent_t e0) {
+    if (!e0) return;
+    hipEvent_t e1 = pool_get(ctx);
+    if (!e1) return;
+    (void)hipEventRecord(e1, ctx->stream);
+    ctx->pending.push_back({what, e0, e1});
+    if (ctx->pending.size() > 4096) prof_resolve(ctx);
+}
+
+int prof_resolve(hd_ctx *ctx) {
+    for (auto &p : ctx->pending) {
+        float ms = 0;
+        HD_HIP(ctx, hipEventSynchronize(p.e1));
+        HD_HIP(ctx, hipEventElapsedTime(&ms, p.e0, p.e1));
+        auto &st = ctx->stage_stats[p.name];
+        st.first += 1;
+        st.second += ms;
+        ctx->event_pool.push_back(p.e0);
+        ctx->event_pool.push_back(p.e1);
+    }
+    ctx->pending.clear();
+    return HD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stage sequences.  `S` is the device descriptor array, `n` the number of
+// scenarios it holds (all of the same shape as `s`).
+// ---------------------------------------------------------------------------
+int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n) {
+    hipStream_t st = ctx->stream;
+    LAUNCH(ctx, "k_segment_rows", k_segment_rows<<<dim3((s.N - 1 + 3) / 4, n), 256, 0, st>>>(S));
+    LAUNCH(ctx, "k_segment_cols", k_segment_cols<<<dim3((s.M - 1 + 63) / 64, n), 64, 0, st>>>(S));
+    LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
+    return HD_OK;
+}
+
+int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n) {
+    hipStream_t st = ctx->stream;
+    Launch2D g = grid2d(s, n);
+    LAUNCH(ctx, "k_onetime_init", k_onetime_init<<<g.grid, g.block, 0, st>>>(S));
+    int rc = run_segmentation(ctx, S, s, n);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_htuongdoi", k_htuongdoi<<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_preprocess", k_preprocess<<<n, 64, 0, st>>>(S));
+    return HD_OK;
+}
+
+int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
+    hipStream_t st = ctx->stream;
+    const hd_options &ops = ctx->ops;
+    Launch2D g = grid2d(s, n);
+    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S, t));
+    int start = 2, end = s.M;
+    if (ops.channel && ops.kenhhepd) {
+        start = 3;
+        end = s.M - 1;
+    }
+    int rc = run_sweep(ctx, S, s, n, 1, start, end);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_smooth<u>", k_smooth<true><<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_solve_v", k_solve_v<<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_margin<v>", k_margin<0><<<dim3((s.N - 1 + 63) / 64, n), 64, 0, st>>>(S));
+    LAUNCH(ctx, "k_smooth<v>", k_smooth<false><<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_post_a", k_post_a<<<g.grid, g.block, 0, st>>>(S));
+    // khouot is unchanged in this half-step, so the segment tables are too;
+    // only the FS boundary interpolation of Find_Calculation_limits runs.
+    LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
+    if (ctx->cfg.sediment && t >= ops.sediment_start) {
+        LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
+        LAUNCH(ctx, "k_fs_sweep<j>",
+               k_fs_sweep<1><<<dim3((s.M - 3 + 63) / 64, n), 64, 0, st>>>(S, 3, s.M - 1, t, ops.bed_change_start));
+        LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
+    }
+    return HD_OK;
+}
+
+int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
+    hipStream_t st = ctx->stream;
+    const hd_options &ops = ctx->ops;
+    Launch2D g = grid2d(s, n);
+    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S, t));
+    int start = 2, end = s.N;
+    if (ops.channel && ops.kenhhepng) start = 3;
+    int rc = run_sweep(ctx, S, s, n, 0, start, end);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_smooth<v>", k_smooth<false><<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_solve_u", k_solve_u<<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_margin<u>", k_margin<1><<<dim3((s.M - 1 + 63) / 64, n), 64, 0, st>>>(S));
+    LAUNCH(ctx, "k_smooth<u>", k_smooth<true><<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_pre_reset", k_pre_reset<<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_rewet", k_rewet<<<dim3((s.M + 63) / 64, n), 64, 0, st>>>(S));
+    LAUNCH(ctx, "k_dry", k_dry<<<g.grid, g.block, 0, st>>>(S));
+    LAUNCH(ctx, "k_post_b", k_post_b<<<g.grid, g.block, 0, st>>>(S));
+    rc = run_segmentation(ctx, S, s, n);
+    if (rc) return rc;
+    if (ctx->cfg.sediment && t >= ops.sediment_start) {
+        LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
+        LAUNCH(ctx, "k_fs_sweep<i>",
+               k_fs_sweep<0><<<dim3((s.N - 3 + 63) / 64, n), 64, 0, st>>>(S, 3, s.N - 1, t, ops.bed_change_start));
+        LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
+        if (ctx->cfg.bed_change && t >= ops.bed_change_start) {
+            LAUNCH(ctx, "k_qb", k_qb<<<g.grid, g.block, 0, st>>>(S));
+            LAUNCH(ctx, "k_bedload", k_bedload<<<g.grid, g.block, 0, st>>>(S));
+        }
+    }
+    return HD_OK;
+}
+
+int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end) {
+    hipStream_t st = ctx->stream;
+    const int lines = end - start + 1;
+    if (lines <= 0) return HD_OK;
+    if (ctx->cfg.form == HD_FORM_FP64_PLAIN) {
+        if (dir == 1)
+            LAUNCH(ctx, "k_sweep_plain<uz>", k_sweep_plain<1><<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, start, end));
+        else
+            LAUNCH(ctx, "k_sweep_plain<vz>", k_sweep_plain<0><<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, start, end));
+        return HD_OK;
+    }
+    return run_sweep_fast(ctx, S, s, n, dir, start, end);
+}
+
+}  // namespace hd
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int hd_finalize_coeffs(hd_coeffs *ce) {
+    if (!ce) return HD_E_INVALID;
+    const double PI = 3.14159265358979323846;
+    ce->dXbp = ce->dX * ce->dX;
+    ce->dX2 = 2 * ce->dX;
+    ce->dYbp = ce->dY * ce->dY;
+    ce->dY2 = 2 * ce->dY;
+    ce->dTchia2dX = ce->dT / ce->dX2;
+    ce->dTchia2dY = ce->dT / ce->dY2;
+    ce->QuyDoiTime = 1.0 / 3600;
+    ce->QuyDoiPi = 1.0 / PI;
+    ce->HaiChiadT = 2.0 / ce->dT;
+    ce->Ks = 2.5 * ce->dm;
+    const double aw = std::fabs(ce->Wind);
+    ce->Windx = 0.0013 * (0.00075 + 0.000067 * aw) * aw * ce->Wind * std::cos(ce->huonggio * (PI / 180));
+    ce->Windy = 0.0013 * (0.00075 + 0.000067 * aw) * aw * ce->Wind * std::sin(ce->huonggio * (PI / 180));
+    ce->Ufr = 0.25 * std::pow((ce->Sx - 1) * ce->g, 8.0 / 15.0) * std::pow(ce->dm, 9.0 / 15.0) *
+              std::pow(ce->muy, -1.0 / 15.0);
+    // loader.cpp:85 evaluates g*(Sx-1)/muy*muy left to right; kept as written
+    ce->Dxr = ce->dm * std::pow(ce->g * (ce->Sx - 1) / ce->muy * ce->muy, (1.0 / 3.0));
+    ce->wss = 10 * ce->muy / ce->dm *
+              (std::sqrt(1 + 0.01 * (ce->Sx - 1) * 9.81 * std::pow(ce->dm, 3) / std::pow(ce->muy, 2)) - 1);
+    return HD_OK;
+}
+
+int hd_load_coeffs(hd_coeffs *ce) {
+    if (!ce) return HD_E_INVALID;
+    std::memset(ce, 0, sizeof(*ce));
+    // loader.cpp:6-78
+    ce->dX = 10.0;
+    ce->dY = 10.0;
+    ce->dT = 2;
+    ce->NANGDAY = 0.0;
+    ce->H_TINH = 0.02;
+    ce->heso = 0.94;
+    ce->mu_mn = 0.2;
+    ce->NDbphai = ce->NDbtrai = ce->NDbtren = ce->NDbduoi = 0.5;
+    ce->Tod = 0.06;
+    ce->Toe = 0.15;
+    ce->ghtoe = 3;
+    ce->Mbochat = 0.00001;
+    ce->ro = 1000;
+    ce->ros = 2000;
+    ce->dm = 0.00001;
+    ce->d90 = 0.002;
+    ce->muy = 1.01e-06;
+    ce->Dorong = 0.5;
+    ce->Sx = 2;
+    ce->sohn = 8;
+    ce->soha = 3;
+    ce->sohn1 = 3;
+    ce->g = 9.81;
+    return hd_finalize_coeffs(ce);
+}
+
+int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops, const hd_config *cfg,
+              hd_ctx **out) {
+    if (!out) return set_err(nullptr, HD_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!mesh || !coeffs || !ops) return set_err(nullptr, HD_E_INVALID, "mesh/coeffs/ops is NULL");
+    if (mesh->M < 4 || mesh->N < 4 || !mesh->h || !mesh->hsnham || !mesh->VISCOIDX)
+        return set_err(nullptr, HD_E_INVALID, "mesh needs M, N >= 4 and h, hsnham, VISCOIDX");
+    if (mesh->total_time < 2) return set_err(nullptr, HD_E_INVALID, "boundary series need >= 2 samples");
+    for (int s = 0; s < 4; s++)
+        if (!mesh->bc[s]) return set_err(nullptr, HD_E_INVALID, "boundary series bc[side] is NULL");
+    const int form = cfg ? cfg->form : HD_FORM_FP64_PLAIN;
+    if (form < HD_FORM_FP64_PLAIN || form > HD_FORM_FP32_REFINE)
+        return set_err(nullptr, HD_E_INVALID, "unknown form");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return set_err(nullptr, HD_E_NODEV, "no HIP device");
+    const int dev = cfg ? cfg->device : 0;
+    if (dev < 0 || dev >= ndev) return set_err(nullptr, HD_E_NODEV, "device ordinal out of range");
+
+    hd_ctx *ctx = new hd_ctx();
+    ctx->device = dev;
+    ctx->c = *coeffs;
+    ctx->ops = *ops;
+    ctx->ops.M = mesh->M;
+    ctx->ops.N = mesh->N;
+    if (cfg) ctx->cfg = *cfg;
+    ctx->cfg.form = form;
+    if (hipSetDevice(dev) != hipSuccess) {
+        delete ctx;
+        return set_err(nullptr, HD_E_NODEV, "hipSetDevice failed");
+    }
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return set_err(nullptr, HD_E_HIP, "hipStreamCreate failed");
+    }
+    int rc = hd::alloc_scenario(ctx, mesh);
+    if (rc) {
+        g_err = ctx->err;
+        hd_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return HD_OK;
+}
+
+int hd_init(hd_ctx *ctx) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    HD_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = run_init(ctx, ctx->ds, ctx->hs, 1);
+    if (rc) return rc;
+    ctx->inited = true;
+    return HD_OK;
+}
+
+int hd_update_boundary(hd_ctx *ctx, double t) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    LAUNCH(ctx, "k_boundary", k_boundary<<<1, 1024, 0, ctx->stream>>>(ctx->ds, t));
+    return HD_OK;
+}
+
+int hd_halfstep_uz(hd_ctx *ctx, double t) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    if (ctx->fault_stage == 1) return set_err(ctx, HD_E_FAULT, "injected fault in UZ half-step");
+    return run_halfstep_uz(ctx, ctx->ds, ctx->hs, 1, t);
+}
+
+int hd_halfstep_vz(hd_ctx *ctx, double t) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    if (ctx->fault_stage == 2) return set_err(ctx, HD_E_FAULT, "injected fault in VZ half-step");
+    return run_halfstep_vz(ctx, ctx->ds, ctx->hs, 1, t);
+}
+
+int hd_step(hd_ctx *ctx, double *t) {
+    if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
+    const double dT = ctx->c.dT;
+    *t += 0.5 * dT;
+    int rc = hd_halfstep_uz(ctx, *t);
+    if (rc) return rc;
+    *t += dT * 0.5;
+    return hd_halfstep_vz(ctx, *t);
+}
+
+int hd_run(hd_ctx *ctx, double *t, double Tmax) {
+    if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
+    const int T = (int)Tmax;
+    while (*t < T) {
+        int rc = hd_step(ctx, t);
+        if (rc) return rc;
+    }
+    return HD_OK;
+}
+
+int hd_sync(hd_ctx *ctx) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_copy_out(hd_ctx *ctx, int field, double *dst, size_t n) {
+    if (!ctx || !dst) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    double *src = field_ptr(ctx, field);
+    const size_t W = ctx->hs.M + 3, H = ctx->hs.N + 3;
+    if (!src || n != W * H) return set_err(ctx, HD_E_INVALID, "bad field id or size");
+    HD_HIP(ctx, hipMemcpy2DAsync(dst, W * sizeof(double), src, ctx->hs.P * sizeof(double), W * sizeof(double), H,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_copy_in(hd_ctx *ctx, int field, const double *src, size_t n) {
+    if (!ctx || !src) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    double *dst = field_ptr(ctx, field);
+    const size_t W = ctx->hs.M + 3, H = ctx->hs.N + 3;
+    if (!dst || n != W * H) return set_err(ctx, HD_E_INVALID, "bad field id or size");
+    HD_HIP(ctx, hipMemcpy2DAsync(dst, ctx->hs.P * sizeof(double), src, W * sizeof(double), W * sizeof(double), H,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_copy_out_i32(hd_ctx *ctx, int field, int32_t *dst, size_t n) {
+    if (!ctx || !dst || field != HD_F_KHOUOT) return set_err(ctx, HD_E_INVALID, "bad argument");
+    const size_t W = ctx->hs.M + 3, H = ctx->hs.N + 3;
+    if (n != W * H) return set_err(ctx, HD_E_INVALID, "bad size");
+    HD_HIP(ctx, hipMemcpy2DAsync(dst, W * sizeof(int32_t), ctx->hs.khouot, ctx->hs.P * sizeof(int32_t),
+                                 W * sizeof(int32_t), H, hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_copy_in_i32(hd_ctx *ctx, int field, const int32_t *src, size_t n) {
+    if (!ctx || !src || field != HD_F_KHOUOT) return set_err(ctx, HD_E_INVALID, "bad argument");
+    const size_t W = ctx->hs.M + 3, H = ctx->hs.N + 3;
+    if (n != W * H) return set_err(ctx, HD_E_INVALID, "bad size");
+    HD_HIP(ctx, hipMemcpy2DAsync(ctx->hs.khouot, ctx->hs.P * sizeof(int32_t), src, W * sizeof(int32_t),
+                                 W * sizeof(int32_t), H, hipMemcpyHostToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+static double *boundary_ptr(hd_ctx *ctx, int which, size_t *n) {
+    const size_t M = ctx->hs.M, N = ctx->hs.N;
+    switch (which) {
+        case HD_B_UBT: *n = M + 3; return ctx->hs.ubt;
+        case HD_B_UBP: *n = M + 3; return ctx->hs.ubp;
+        case HD_B_VBT: *n = N + 3; return ctx->hs.vbt;
+        case HD_B_VBD: *n = N + 3; return ctx->hs.vbd;
+        case HD_B_HI: *n = 2 * (M + N + 6); return ctx->hs.hi;
+        default: *n = 0; return nullptr;
+    }
+}
+
+int hd_copy_out_boundary(hd_ctx *ctx, int which, double *dst, size_t n) {
+    if (!ctx || !dst) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    size_t want;
+    double *src = boundary_ptr(ctx, which, &want);
+    if (!src || n != want) return set_err(ctx, HD_E_INVALID, "bad boundary id or size");
+    HD_HIP(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_copy_in_boundary(hd_ctx *ctx, int which, const double *src, size_t n) {
+    if (!ctx || !src) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    size_t want;
+    double *dst = boundary_ptr(ctx, which, &want);
+    if (!dst || n != want) return set_err(ctx, HD_E_INVALID, "bad boundary id or size");
+    HD_HIP(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_get_hmax(hd_ctx *ctx, double out[4]) {
+    if (!ctx || !out) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    HD_HIP(ctx, hipMemcpyAsync(out, ctx->hs.hmax, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_set_hmax(hd_ctx *ctx, const double in[4]) {
+    if (!ctx || !in) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    HD_HIP(ctx, hipMemcpyAsync(ctx->hs.hmax, in, 4 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_get_segments(hd_ctx *ctx, int dir, int32_t *counts, size_t ncounts, int32_t *first, int32_t *last,
+                    size_t cap, int64_t *total) {
+    if (!ctx || (dir != 0 && dir != 1)) return set_err(ctx, HD_E_INVALID, "bad argument");
+    const DevState &s = ctx->hs;
+    const size_t nlines = dir == 0 ? (size_t)s.N + 3 : (size_t)s.M + 3;
+    const size_t lcap = dir == 0 ? s.capH : s.capV;
+    if (counts && ncounts != nlines) return set_err(ctx, HD_E_INVALID, "counts size must be the line count");
+    std::vector<int32_t> n(nlines), f(nlines * lcap), l(nlines * lcap);
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    HD_HIP(ctx, hipMemcpy(n.data(), dir == 0 ? s.segH_n : s.segV_n, nlines * 4, hipMemcpyDeviceToHost));
+    HD_HIP(ctx, hipMemcpy(f.data(), dir == 0 ? s.segH_f : s.segV_f, nlines * lcap * 4, hipMemcpyDeviceToHost));
+    HD_HIP(ctx, hipMemcpy(l.data(), dir == 0 ? s.segH_l : s.segV_l, nlines * lcap * 4, hipMemcpyDeviceToHost));
+    // lines outside [2, N] / [2, M] carry no segments
+    const int lo = 2, hi = dir == 0 ? s.N : s.M;
+    int64_t k = 0;
+    for (size_t line = 0; line < nlines; line++) {
+        const int cnt = ((int)line >= lo && (int)line <= hi) ? n[line] : 0;
+        if (counts) counts[line] = cnt;
+        for (int q = 0; q < cnt; q++, k++)
+            if ((size_t)k < cap && first && last) {
+                first[k] = f[line * lcap + q];
+                last[k] = l[line * lcap + q];
+            }
+    }
+    if (total) *total = k;
+    return HD_OK;
+}
+
+int hd_mass_balance(hd_ctx *ctx, double *volume, double *inflow) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    LAUNCH(ctx, "k_volume", k_volume<<<1, 1024, 0, ctx->stream>>>(ctx->ds));
+    double acc[2];
+    HD_HIP(ctx, hipMemcpyAsync(acc, ctx->hs.acc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (inflow) *inflow = acc[0];
+    if (volume) *volume = acc[1];
+    return HD_OK;
+}
+
+int hd_destroy(hd_ctx *ctx) {
+    if (!ctx) return HD_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    prof_resolve(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return HD_OK;
+}
+
+const char *hd_last_error(hd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int hd_set_profiling(hd_ctx *ctx, int on) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    int rc = prof_resolve(ctx);
+    if (rc) return rc;
+    ctx->prof = on != 0;
+    return HD_OK;
+}
+
+int hd_stage_stats(hd_ctx *ctx, char *buf, size_t n, int reset) {
+    if (!ctx || !buf || n == 0) return set_err(ctx, HD_E_INVALID, "bad argument");
+    int rc = prof_resolve(ctx);
+    if (rc) return rc;
+    std::string out = "{";
+    bool firstk = true;
+    for (auto &kv : ctx->stage_stats) {
+        char item[256];
+        std::snprintf(item, sizeof(item), "%s\"%s\": [%ld, %.6f]", firstk ? "" : ", ", kv.first.c_str(),
+                      kv.second.first, kv.second.second);
+        out += item;
+        firstk = false;
+    }
+    out += "}";
+    if (out.size() + 1 > n) return set_err(ctx, HD_E_INVALID, "buffer too small");
+    std::memcpy(buf, out.c_str(), out.size() + 1);
+    if (reset) ctx->stage_stats.clear();
+    return HD_OK;
+}
+
+int hd_device_info(char *buf, size_t n) {
+    if (!buf || n == 0) return HD_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        std::snprintf(buf, n, "no HIP device");
+        return HD_E_NODEV;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return HD_E_HIP;
+    std::snprintf(buf, n, "%s %s CUs=%d L2=%d MB HBM=%.1f GB", p.name, p.gcnArchName, p.multiProcessorCount,
+                  p.l2CacheSize >> 20, p.totalGlobalMem / 1e9);
+    return HD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// hd_context.hip - device allocation and upload of one scenario
+// (attribute_arrays_memory_alloc / supporting_arrays_alloc,
+// loader.cpp:224-375, with every buffer zero-initialised).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "hd_internal.h"
+
+namespace hd {
+
+namespace {
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int upload2d(hd_ctx *ctx, double *dst, const double *src, const DevState &s) {
+    if (!src) return HD_OK;
+    const size_t W = s.M + 3, H = s.N + 3;
+    hipError_t e = hipMemcpy2D(dst, s.P * sizeof(double), src, W * sizeof(double), W * sizeof(double), H,
+                               hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ctx->err = std::string("upload: ") + hipGetErrorString(e);
+        return HD_E_HIP;
+    }
+    return HD_OK;
+}
+
+}  // namespace
+
+int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
+    DevState &s = ctx->hs;
+    s.M = mesh->M;
+    s.N = mesh->N;
+    s.P = round_up(s.M + 3, 32);
+    s.H = s.N + 4;
+    s.ldT = round_up(s.N + 3, 32);
+    s.capH = (s.M + 3) / 2 + 2;
+    s.capV = (s.N + 3) / 2 + 2;
+    s.total_time = mesh->total_time;
+    for (int k = 0; k < 4; k++) {
+        s.btype[k] = mesh->boundary_type[k];
+        s.bienQ[k] = mesh->boundary_type[k] == 2 ? 1 : 0;
+        s.hasCC[k] = mesh->CC[k] != nullptr;
+    }
+    s.quirks = ctx->cfg.quirks;
+    s.ketdinh = ctx->ops.ketdinh ? 1 : 0;
+    s.c = ctx->c;
+    const size_t cells = (size_t)s.P * s.H;
+    const size_t fbytes = cells * sizeof(double);
+    int rc = 0;
+    auto F = [&](double **p) {
+        if (!rc) rc = ctx_alloc(ctx, (void **)p, fbytes);
+    };
+    auto A = [&](void **p, size_t bytes) {
+        if (!rc) rc = ctx_alloc(ctx, p, bytes);
+    };
+    double *h, *hsn, *vis, *fw, *tx, *ty;
+    F(&h); F(&hsn); F(&vis); F(&fw); F(&tx); F(&ty);
+    F(&s.htaiz); F(&s.htaiz_bd); F(&s.Kx1); F(&s.Ky1);
+    F(&s.u); F(&s.v); F(&s.z); F(&s.t_u); F(&s.t_v); F(&s.t_z); F(&s.H_moi); F(&s.Htdu); F(&s.Htdv);
+    F(&s.FS); F(&s.tFS); F(&s.VTH); F(&s.Kx); F(&s.Ky); F(&s.Qbx); F(&s.Qby); F(&s.dH); F(&s.tmp);
+    A((void **)&s.khouot, cells * sizeof(int));
+    A((void **)&s.hi, 2 * (size_t)(s.M + s.N + 6) * sizeof(double));
+    A((void **)&s.hmax, 4 * sizeof(double));
+    A((void **)&s.ubt, (s.M + 3) * sizeof(double));
+    A((void **)&s.ubp, (s.M + 3) * sizeof(double));
+    A((void **)&s.vbt, (s.N + 3) * sizeof(double));
+    A((void **)&s.vbd, (s.N + 3) * sizeof(double));
+    for (int k = 0; k < 4; k++) {
+        double *b = nullptr, *cc = nullptr;
+        A((void **)&b, s.total_time * sizeof(double));
+        A((void **)&cc, s.total_time * sizeof(double));
+        s.bc[k] = b;
+        s.CC[k] = cc;
+    }
+    A((void **)&s.segH_n, (s.N + 3) * sizeof(int));
+    A((void **)&s.segH_f, (size_t)(s.N + 3) * s.capH * sizeof(int));
+    A((void **)&s.segH_l, (size_t)(s.N + 3) * s.capH * sizeof(int));
+    A((void **)&s.segV_n, (s.M + 3) * sizeof(int));
+    A((void **)&s.segV_f, (size_t)(s.M + 3) * s.capV * sizeof(int));
+    A((void **)&s.segV_l, (size_t)(s.M + 3) * s.capV * sizeof(int));
+    const size_t scratch = std::max((size_t)2 * s.H * s.P, (size_t)2 * (s.M + 4) * s.ldT);
+    A((void **)&s.Ap, scratch * sizeof(double));
+    A((void **)&s.Bp, scratch * sizeof(double));
+    A((void **)&s.acc, (8 + (size_t)s.H) * sizeof(double));
+    A((void **)&ctx->ds, sizeof(DevState));
+    if (rc) return rc;
+    s.h = h; s.hsnham = hsn; s.visc = vis; s.Fw = fw; s.Tsxw = tx; s.Tsyw = ty;
+    if ((rc = upload2d(ctx, h, mesh->h, s))) return rc;
+    if ((rc = upload2d(ctx, hsn, mesh->hsnham, s))) return rc;
+    if ((rc = upload2d(ctx, vis, mesh->VISCOIDX, s))) return rc;
+    if ((rc = upload2d(ctx, fw, mesh->Fw, s))) return rc;
+    if ((rc = upload2d(ctx, tx, mesh->Tsxw, s))) return rc;
+    if ((rc = upload2d(ctx, ty, mesh->Tsyw, s))) return rc;
+    if ((rc = upload2d(ctx, s.u, mesh->u0, s))) return rc;
+    if ((rc = upload2d(ctx, s.v, mesh->v0, s))) return rc;
+    if ((rc = upload2d(ctx, s.z, mesh->z0, s))) return rc;
+    if ((rc = upload2d(ctx, s.FS, mesh->FS0, s))) return rc;
+    for (int k = 0; k < 4; k++) {
+        if (hipMemcpy((void *)s.bc[k], mesh->bc[k], s.total_time * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return HD_E_HIP;
+        if (mesh->CC[k] && hipMemcpy((void *)s.CC[k], mesh->CC[k], s.total_time * sizeof(double),
+                                     hipMemcpyHostToDevice) != hipSuccess)
+            return HD_E_HIP;
+    }
+    if (hipMemcpy(ctx->ds, &s, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) return HD_E_HIP;
+    return HD_OK;
+}
+
+// Fast forms are added separately; until then they report unsupported.
+int run_sweep_fast(hd_ctx *ctx, const DevState *, const DevState &, int, int, int, int) {
+    ctx->err = "fast sweep forms are not built";
+    return HD_E_UNSUPPORTED;
+}
+
+}  // namespace hd

@@ -1,0 +1,349 @@
+// hd_device.h - device-side data layout and per-cell arithmetic of the
+// Hydist_C ADI timestep on gfx950.
+//
+// Every expression below is evaluated in the reference's operand order and
+// the library is built with -ffp-contract=off, so values formed from
+// + - * / sqrt are bitwise those of the reference's FP64 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hydist.h"
+
+#define HD_DEV __device__ __forceinline__
+
+namespace hd {
+
+// One scenario's device buffers.  2-D fields are row-major [i][j] with
+// pitch P (elements, a multiple of 32 so every row starts 256-B aligned);
+// rows 0..N+2 are meaningful, row N+3 is spare.
+struct DevState {
+    int M, N, P, H;        // H = allocated rows (N + 4)
+    int capH, capV;        // per-line capacity of the segment tables
+    int ldT;               // leading dimension of the row-sweep scratch (>= N + 3)
+    int total_time;
+    int btype[4], bienQ[4];
+    int hasCC[4];
+    int quirks, ketdinh;
+    hd_coeffs c;
+    // inputs
+    const double *h, *hsnham, *visc, *Fw, *Tsxw, *Tsyw;
+    const double *bc[4], *CC[4];
+    // derived once at init
+    double *htaiz, *htaiz_bd, *Kx1, *Ky1, *hi, *hmax;
+    // state
+    double *u, *v, *z, *t_u, *t_v, *t_z, *H_moi, *Htdu, *Htdv, *FS, *tFS, *VTH, *Kx, *Ky, *Qbx,
+        *Qby, *dH;
+    int *khouot;
+    double *ubt, *ubp, *vbt, *vbd;
+    // wet-segment tables: rows (along j) and columns (along i)
+    int *segH_n, *segH_f, *segH_l;
+    int *segV_n, *segV_f, *segV_l;
+    // scratch
+    double *tmp;       // smoothing output (full field)
+    double *Ap, *Bp;   // Thomas sweep scratch, two rows per cell
+    double *acc;       // [0] boundary inflow volume, [1] mass-balance result
+};
+
+HD_DEV int at(const DevState &s, int i, int j) { return i * s.P + j; }
+
+// ---------------------------------------------------------------------------
+// Coefficients of one cell of an implicit hydraulic line
+// (_uzSolver_calculate_preindex, UVZSolver_multithread.cpp:210-278;
+//  _vzSolver_calculate_preindex, :142-208).
+// ---------------------------------------------------------------------------
+struct Pre {
+    double f1, f2, f3, f5, a2, c2, d2;
+};
+
+HD_DEV Pre uz_preindex(const DevState &s, int i, int j) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Htdu = s.Htdu, *Htdv = s.Htdv, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Kx1 = s.Kx1, *Tsxw = s.Tsxw;
+    const double dT = c.dT, dXbp = c.dXbp, dYbp = c.dYbp, dY2 = c.dY2, dTchia2dY = c.dTchia2dY,
+                 dTchia2dX = c.dTchia2dX, H_TINH = c.H_TINH;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    const int ij = i * W + j;
+    const double uij = u[ij];
+    double vtb = (v[i * W + j - 1] + v[ij] + v[(i + 1) * W + j - 1] + v[(i + 1) * W + j]) * 0.25;
+    r.f1 = dTchia2dX * uij + VIS[ij] * dT / dXbp;
+    r.f2 = -(2 + Kx1[ij] * dT * sqrt(uij * uij + vtb * vtb) / Htdu[ij] + (2 * dT * VIS[ij]) / dXbp);
+    r.f3 = dT * VIS[ij] / dXbp - dTchia2dX * uij;
+    if (H_moi[ij - 1] <= H_TINH) {
+        if (vtb < 0) {
+            p = vtb * (-3 * uij + 4 * u[ij + 1] - u[ij + 2]) / dY2;
+            q = (uij - 2 * u[ij + 1] + u[ij + 2]) / dYbp;
+        }
+    } else {
+        if (H_moi[ij + 1] <= H_TINH) {
+            if ((H_moi[ij - 2] > H_TINH) && (vtb > 0)) {
+                p = vtb * (3 * uij - 4 * u[ij - 1] + u[ij - 2]) / dY2;
+                q = (uij - 2 * u[ij - 1] + u[ij - 2]) / dYbp;
+            }
+        } else {
+            p = vtb * (u[ij + 1] - u[ij - 1]) / dY2;
+            q = (u[ij + 1] - 2 * uij + u[ij - 1]) / dYbp;
+        }
+    }
+    r.f5 = -2 * uij + dT * p - dT * c.CORIOLIS_FORCE * vtb - dT * VIS[ij] * q -
+           dT * (c.Windx - Tsxw[ij]) / Htdu[ij];
+    r.c2 = dTchia2dX * Htdu[ij];
+    r.a2 = -dTchia2dX * Htdu[(i - 1) * W + j];
+    r.d2 = z[ij] - dTchia2dY * (Htdv[ij] * v[ij] - Htdv[ij - 1] * v[ij - 1]);
+    return r;
+}
+
+HD_DEV Pre vz_preindex(const DevState &s, int i, int j) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Htdu = s.Htdu, *Htdv = s.Htdv, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Ky1 = s.Ky1, *Tsyw = s.Tsyw;
+    const double dT = c.dT, dXbp = c.dXbp, dYbp = c.dYbp, dX2 = c.dX2, dTchia2dY = c.dTchia2dY,
+                 dTchia2dX = c.dTchia2dX, H_TINH = c.H_TINH;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    const int ij = i * W + j;
+    const double vij = v[ij];
+    double utb = (u[(i - 1) * W + j] + u[ij] + u[(i - 1) * W + j + 1] + u[ij + 1]) * 0.25;
+    r.f1 = dTchia2dY * vij + VIS[ij] * dT / dYbp;
+    r.f2 = -(2 + Ky1[ij] * dT * sqrt(vij * vij + utb * utb) / Htdv[ij] + (2 * dT * VIS[ij]) / dYbp);
+    r.f3 = dT * VIS[ij] / dYbp - dTchia2dY * vij;
+    if (H_moi[(i - 1) * W + j] <= H_TINH) {
+        if (utb < 0) {
+            q = utb * (-3 * vij + 4 * v[(i + 1) * W + j] - v[(i + 2) * W + j]) / dX2;
+            p = (vij - 2 * v[(i + 1) * W + j] + v[(i + 2) * W + j]) / dXbp;
+        }
+    } else {
+        if (H_moi[(i + 1) * W + j] <= H_TINH) {
+            if ((H_moi[(i - 2) * W + j] > H_TINH) && (utb > 0)) {
+                q = utb * (3 * vij - 4 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / dX2;
+                p = (vij - 2 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) / dXbp;
+            }
+        } else {
+            q = utb * (v[(i + 1) * W + j] - v[(i - 1) * W + j]) / dX2;
+            p = (v[(i + 1) * W + j] - 2 * vij + v[(i - 1) * W + j]) / dXbp;
+        }
+    }
+    r.f5 = -2 * vij + dT * q + dT * c.CORIOLIS_FORCE * utb - dT * VIS[ij] * p -
+           dT * (c.Windy - Tsyw[ij]) / Htdv[ij];
+    r.c2 = dTchia2dY * Htdv[ij];
+    r.a2 = -dTchia2dY * Htdv[ij - 1];
+    r.d2 = z[ij] - dTchia2dX * (Htdu[ij] * u[ij] - Htdu[(i - 1) * W + j] * u[(i - 1) * W + j]);
+    return r;
+}
+
+// Momentum row after continuity is substituted (_calculate_abcd,
+// UVZSolver_multithread.cpp:280-342) at position p of segment [f, l]:
+// interior formula, ran1 override at f, ran2 override at l-1, or the short
+// form for segments of one or two cells.  `cur` is the cell's
+// coefficients, `nxt` those of p+1 (unused by the short form and the ran2
+// override).
+struct Mom {
+    double a1, b1, c1, d1;
+};
+
+HD_DEV Mom calc_abcd(int p, int f, int l, double f4, bool ran1, bool ran2, const Pre &cur,
+                     const Pre &nxt) {
+    Mom m;
+    if (l - 1 > f) {
+        if (ran1 && p == f) {
+            m.a1 = f4;
+            m.b1 = cur.f2 - (cur.f3 * nxt.a2 / nxt.c2);
+            m.c1 = -f4 - (cur.f3 / nxt.c2);
+            m.d1 = cur.f5 - (cur.f3 * nxt.d2 / nxt.c2);
+        } else if (ran2 && p == l - 1) {
+            m.a1 = f4 - cur.f1 / cur.a2;
+            m.b1 = cur.f2 - (cur.f1 * cur.c2 / cur.a2);
+            m.c1 = -f4;
+            m.d1 = cur.f5 - (cur.f1 * cur.d2 / cur.a2);
+        } else {
+            m.a1 = f4 - cur.f1 / cur.a2;
+            m.c1 = -f4 - (cur.f3 / nxt.c2);
+            m.b1 = cur.f2 - (cur.f1 * cur.c2 / cur.a2) - (cur.f3 * nxt.a2 / nxt.c2);
+            m.d1 = cur.f5 - (cur.f1 * cur.d2 / cur.a2) - (cur.f3 * nxt.d2 / nxt.c2);
+        }
+    } else {
+        m.a1 = f4;
+        m.b1 = cur.f2;
+        m.d1 = cur.f5;
+        m.c1 = -f4;
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Explicit momentum of the non-swept component
+// (vSolver, UVZSolver_multithread.cpp:607-663; uSolver :724-795).
+// ---------------------------------------------------------------------------
+HD_DEV double v_solver(const DevState &s, const double *__restrict__ t_u, int row, int col) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Ky1 = s.Ky1, *Htdv = s.Htdv, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Tsyw = s.Tsyw;
+    const double H_TINH = c.H_TINH, dY = c.dY, dY2 = c.dY2, dX2 = c.dX2, dXbp = c.dXbp,
+                 dYbp = c.dYbp, g = c.g, HaiChiadT = c.HaiChiadT;
+    const int rc = row * W + col;
+    double q = 0, p = 0, tmp = 0;
+    const double vrc = v[rc];
+    double utb = (u[(row - 1) * W + col] + u[rc] + u[(row - 1) * W + col + 1] + u[rc + 1]) * 0.25;
+    double t_utb =
+        (t_u[(row - 1) * W + col] + t_u[rc] + t_u[(row - 1) * W + col + 1] + t_u[rc + 1]) * 0.25;
+    p = (v[rc + 1] - v[rc - 1]) / dY2;
+    p = HaiChiadT + p + Ky1[rc] * sqrt(utb * utb + vrc * vrc) / Htdv[rc];
+    if (H_moi[(row - 1) * W + col] <= H_TINH) {
+        if (utb < 0) {
+            q = t_utb * (-3 * vrc + 4 * v[(row + 1) * W + col] - v[(row + 2) * W + col]) / dX2;
+            tmp = (vrc - 2 * v[(row + 1) * W + col] + v[(row + 2) * W + col]) / dXbp;
+        }
+    } else {
+        if (H_moi[(row + 1) * W + col] <= H_TINH) {
+            if ((H_moi[(row - 2) * W + col] > H_TINH) && (utb > 0)) {
+                q = t_utb * (3 * vrc - 4 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / dX2;
+                tmp = (vrc - 2 * v[(row - 1) * W + col] + v[(row - 2) * W + col]) / dXbp;
+            }
+        } else {
+            q = t_utb * (v[(row + 1) * W + col] - v[(row - 1) * W + col]) / dX2;
+            tmp = (v[(row + 1) * W + col] - 2 * vrc + v[(row - 1) * W + col]) / dXbp;
+        }
+    }
+    q = HaiChiadT * vrc - q - c.CORIOLIS_FORCE * t_utb;
+    q = (q - g * (z[rc + 1] - z[rc]) / dY + VIS[rc] * (tmp + (v[rc + 1] - 2 * vrc + v[rc - 1]) / dYbp)) +
+        (c.Windy - Tsyw[rc]) / Htdv[rc];
+    return q / p;
+}
+
+// uSolver takes dX from dY and Windx from Windy (UVZSolver_multithread.cpp:731,734).
+HD_DEV double u_solver(const DevState &s, const double *__restrict__ t_v, int row, int col) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Kx1 = s.Kx1, *Htdu = s.Htdu, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Tsxw = s.Tsxw;
+    const double H_TINH = c.H_TINH, dX = c.dY, dY = c.dY, Windx = c.Windy, g = c.g,
+                 HaiChiadT = c.HaiChiadT;
+    const int i_j = row * W + col;
+    const int i_1_j = i_j + W;
+    double p = 0, q = 0, tmp = 0;
+    const double uij = u[i_j];
+    double vtb = (v[i_j - 1] + v[i_j] + v[i_1_j - 1] + v[i_1_j]) * 0.25;
+    double t_vtb = (t_v[i_j - 1] + t_v[i_j] + t_v[i_1_j - 1] + t_v[i_1_j]) * 0.25;
+    p = (u[i_1_j] - u[(row - 1) * W + col]) / (dX * 2);
+    p = (HaiChiadT + p + Kx1[i_j] * sqrt(vtb * vtb + uij * uij) / Htdu[i_j]);
+    if (H_moi[i_j - 1] <= H_TINH) {
+        if (vtb < 0) {
+            q = t_vtb * (-3 * uij + 4 * u[i_j + 1] - u[i_j + 2]) / (dY * 2);
+            tmp = (uij - 2 * u[i_j + 1] + u[i_j + 2]) / (dY * dY);
+        }
+    } else {
+        if (H_moi[i_j + 1] <= H_TINH) {
+            if ((H_moi[i_j - 2] > H_TINH) && (vtb > 0)) {
+                q = t_vtb * (3 * uij - 4 * u[i_j - 1] + u[i_j - 2]) / (dY * 2);
+                tmp = (uij - 2 * u[i_j - 1] + u[i_j - 2]) / (dY * dY);
+            }
+        } else {
+            q = t_vtb * (u[i_j + 1] - u[i_j - 1]) / (dY * 2);
+            tmp = (u[i_j + 1] - 2 * uij + u[i_j - 1]) / (dY * dY);
+        }
+    }
+    q = HaiChiadT * uij - q + c.CORIOLIS_FORCE * t_vtb;
+    q = (q - g * (z[i_1_j] - z[i_j]) / dX +
+         VIS[i_j] * (tmp + (u[i_1_j] - 2 * uij + u[(row - 1) * W + col]) / (dX * dX))) +
+        (Windx - Tsxw[i_j]) / Htdu[i_j];
+    return q / p;
+}
+
+// ---------------------------------------------------------------------------
+// Sediment cell terms (sediment_transport.cpp:11-49).
+// ---------------------------------------------------------------------------
+HD_DEV double source_chs(const hd_coeffs &c, double wsm, double Zf, double fs, double fw,
+                         double vth, double h_moi) {
+    double toee = c.Toe;
+    double todd = c.Tod;
+    double S = 0;
+    double tob = c.ro * fw * (vth * vth);
+    if (h_moi > c.ghtoe) toee = toee * (1 + c.hstoe * ((h_moi - c.ghtoe)));
+    if (tob > toee)
+        S = c.Mbochat / c.ros * (tob - toee) / toee;
+    else {
+        if (tob < todd) {
+            double Pd = 1 - (tob / todd);
+            double beta = 1 + (Zf / (1.25 + 4.75 * pow(Pd, 2.5)));
+            double Cb = beta * fs;
+            S = -wsm * Cb * Pd;
+        }
+    }
+    return S;
+}
+
+HD_DEV double source_nchs(const hd_coeffs &c, double wsm, double Zf, double Uf, double h_moi,
+                          double fs) {
+    double gamac = 0.434 + 5.975 * Zf + 2.888 * Zf * Zf;
+    double Tx = fmax(0.0, (Uf * Uf - c.Ufr * c.Ufr) / (c.Ufr * c.Ufr));
+    double Cac = 0.015 * c.dm * pow(Tx, 1.5) * pow(c.Dxr, (-0.3)) / (0.05 * h_moi);
+    return wsm * (Cac - gamac * fs);
+}
+
+struct FSCell {
+    double wsm, Zf, S;
+};
+
+// Settling velocity, Zf and source of one cell, as formed at the top of
+// _FSi/_FSj_calculate__mactrix_coeff (sediment_transport.cpp:92-108, :241-265).
+HD_DEV FSCell fs_cell(const DevState &s, int pos, double t, double s_start) {
+    const hd_coeffs &c = s.c;
+    FSCell r;
+    const double H = s.H_moi[pos], fsv = s.FS[pos], vth = s.VTH[pos];
+    double cc = 18 * log(12 * H / c.Ks);
+    double Uf = sqrt(c.g) * fabs(vth) / cc;
+    r.wsm = c.wss * pow((1 - fsv), 4);
+    r.Zf = 0;
+    if (fsv > 0.00001) r.Zf = r.wsm / (0.4 * (Uf + 2 * r.wsm));
+    if (s.ketdinh)
+        r.S = source_chs(c, r.wsm, r.Zf, fsv, s.Fw[pos], vth, H);
+    else
+        r.S = source_nchs(c, r.wsm, r.Zf, Uf, H, fsv);
+    if (t < s_start) r.S = 0;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Segment lookup in a line's sorted segment list: index of the segment
+// containing position p, or -1.
+// ---------------------------------------------------------------------------
+HD_DEV int find_segment(const int *__restrict__ first, const int *__restrict__ last, int n, int p) {
+    int lo = 0, hi = n - 1, k = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        if (first[mid] <= p) {
+            k = mid;
+            lo = mid + 1;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    if (k >= 0 && p <= last[k]) return k;
+    return -1;
+}
+
+// Wall tests of locate_segment_u / locate_segment_v
+// (support_funcs.cpp:823-826, :845-851).  dir 1: line = column j; 0: row i.
+HD_DEV void wall_flags(const DevState &s, int dir, int line, int first, int last, bool &ran1,
+                       bool &ran2) {
+    const int W = s.P;
+    const double ND = s.c.NANGDAY;
+    const double *h = s.h;
+    if (dir == 1) {
+        const int j = line;
+        ran1 = (first > 2) || ((first == 2) && ((h[1 * W + j] + h[1 * W + j - 1]) * 0.5 == ND));
+        ran2 = (last < s.N) || ((last == s.N) && ((h[s.N * W + j] + h[s.N * W + j - 1]) * 0.5 == ND));
+    } else {
+        const int i = line;
+        ran1 = (first > 2) ||
+               ((first == 2) && ((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 == ND));
+        ran2 = (last < s.M) || ((last == s.M) && ((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 == ND));
+    }
+}
+
+}  // namespace hd

@@ -1,0 +1,72 @@
+// hd_internal.h - host-side context and stage sequences shared by the C ABI
+// translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hd_kernels.h"
+
+struct hd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hd_coeffs c{};
+    hd_options ops{};
+    hd_config cfg{};
+    hd::DevState hs{};             // descriptor (host copy)
+    hd::DevState *ds = nullptr;    // descriptor (device copy)
+    std::vector<void *> allocs;
+    size_t bytes = 0;
+    bool inited = false;
+    int fault_stage = 0;           // test hook: 1 = fail UZ half-step, 2 = fail VZ
+    std::string err;
+    // stage profiling: events recorded on `stream` around each launch
+    bool prof = false;
+    std::vector<hipEvent_t> event_pool;
+    struct Pending {
+        const char *name;
+        hipEvent_t e0, e1;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<long, double>> stage_stats;  // name -> (launches, total ms)
+};
+
+namespace hd {
+
+struct Launch2D {
+    dim3 grid, block;
+};
+
+int ctx_alloc(hd_ctx *ctx, void **p, size_t bytes);
+double *field_ptr(hd_ctx *ctx, int f);
+Launch2D grid2d(const DevState &s, int nscen);
+int sync_desc(hd_ctx *ctx);
+int check_launch(hd_ctx *ctx, const char *what);
+hipEvent_t prof_begin(hd_ctx *ctx);
+void prof_end(hd_ctx *ctx, const char *what, hipEvent_t e0);
+int prof_resolve(hd_ctx *ctx);
+
+// Launches a kernel (the variadic part), checks the launch and, when
+// profiling, brackets it with events on the context stream.
+#define LAUNCH(ctx, what, ...)                          \
+    do {                                                \
+        hipEvent_t e0_ = hd::prof_begin(ctx);           \
+        __VA_ARGS__;                                    \
+        int rc_ = hd::check_launch((ctx), (what));      \
+        if (rc_) return rc_;                            \
+        hd::prof_end((ctx), (what), e0_);               \
+    } while (0)
+int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh);
+
+int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
+int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
+int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
+int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
+int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end);
+int run_sweep_fast(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end);
+
+}  // namespace hd

@@ -1,0 +1,1141 @@
+// hd_kernels.hip - gfx950 kernels of the Hydist_C ADI timestep.
+//
+// Layout: every kernel takes `const DevState *S` (an array of scenario
+// descriptors in device memory) and works on scenario blockIdx.z (2-D
+// kernels) or blockIdx.y (line kernels), so one launch can step many
+// scenarios of the same shape.
+#include "hd_kernels.h"
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// Init: Onetime_init (support_funcs.cpp:21-69), per cell.
+// ---------------------------------------------------------------------------
+__global__ void k_onetime_init(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int M = s.M, N = s.N, W = s.P;
+    if (i > N + 3 || j > M + 2) return;
+    const double *h = s.h, *hsn = s.hsnham;
+    int kh = 2;
+    if (i >= 1 && i <= N + 1 && j >= 1 && j <= M + 1) {
+        double sum = (h[(i - 1) * W + j - 1] + h[(i - 1) * W + j] + h[i * W + j - 1] + h[i * W + j]);
+        if (sum * 0.25 > s.c.NANGDAY) {
+            kh = 0;
+            s.H_moi[i * W + j] = 0;
+        }
+        if (i <= N && j <= M) {
+            const double ht = (h[(i - 1) * W + j - 1] + h[(i - 1) * W + j] + h[i * W + j - 1] + h[i * W + j]) * 0.25;
+            s.htaiz[i * W + j] = ht;
+            s.htaiz_bd[i * W + j] = ht;
+            const double g = s.c.g, mu = s.c.mu_mn;
+            if (h[i * W + j - 1] + h[i * W + j] != 0)
+                s.Kx1[i * W + j] = g * pow((h[i * W + j - 1] + h[i * W + j]) * 0.5, -2 * mu) *
+                                   pow((hsn[i * W + j - 1] + hsn[i * W + j]) * 0.5, 2.0);
+            if (h[(i - 1) * W + j] + h[i * W + j] != 0)
+                s.Ky1[i * W + j] = g * pow((h[(i - 1) * W + j] + h[i * W + j]) * 0.5, -2 * mu) *
+                                   pow((hsn[(i - 1) * W + j] + hsn[i * W + j]) * 0.5, 2.0);
+        }
+    }
+    s.khouot[i * W + j] = kh;
+}
+
+// ---------------------------------------------------------------------------
+// Segmentation (Find_Calculation_limits_Horizontal/Vertical,
+// support_funcs.cpp:238-343) as run extraction: a run of wet cells clipped to
+// [2, end] is a segment unless it starts at end-1 or later.  No cap on the
+// number of segments per line.
+// ---------------------------------------------------------------------------
+
+// Rows (lines along j, unit stride): one wavefront per row; each lane tests
+// one cell per chunk and ballots run starts / ends.
+__global__ void __launch_bounds__(256) k_segment_rows(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.y];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wave + 2;
+    if (i > s.N) return;
+    const int end = s.M, W = s.P;
+    const int *kh = s.khouot + i * W;
+    int *first = s.segH_f + (size_t)i * s.capH, *last = s.segH_l + (size_t)i * s.capH;
+    int nstart = 0, nend = 0, last_start = -1;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int base = 2; base <= end; base += 64) {
+        const int p = base + lane;
+        bool st = false, en = false;
+        if (p <= end) {
+            const bool wet = kh[p] == 0;
+            st = wet && (p == 2 || kh[p - 1] != 0);
+            en = wet && (p == end || kh[p + 1] != 0);
+        }
+        const uint64_t bs = __ballot(st), be = __ballot(en);
+        if (st) first[nstart + __popcll(bs & below)] = p;
+        if (en) last[nend + __popcll(be & below)] = p;
+        nstart += __popcll(bs);
+        nend += __popcll(be);
+        if (bs) last_start = base + 63 - __clzll(bs);
+    }
+    // a run starting at end-1 or end is not a segment; it is always the last run
+    int n = nstart;
+    if (n > 0 && last_start >= end - 1) n--;
+    if (lane == 0) s.segH_n[i] = n;
+}
+
+// Columns (lines along i, stride P): one lane per column, lanes of a
+// wavefront read adjacent columns of the same row.
+__global__ void __launch_bounds__(64) k_segment_cols(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    if (j > s.M) return;
+    const int end = s.N, W = s.P;
+    const int *kh = s.khouot;
+    int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
+    int n = 0;
+    bool prev_wet = false;
+    for (int p = 2; p <= end; p++) {
+        const bool wet = kh[p * W + j] == 0;
+        if (wet && !prev_wet) first[n] = p;
+        if (wet && (p == end || kh[(p + 1) * W + j] != 0)) {
+            last[n] = p;
+            n++;
+        }
+        prev_wet = wet;
+    }
+    if (n > 0 && first[n - 1] >= end - 1) n--;
+    s.segV_n[j] = n;
+}
+
+// ---------------------------------------------------------------------------
+// Htuongdoi (support_funcs.cpp:346-362) alone (init).
+// ---------------------------------------------------------------------------
+__global__ void k_htuongdoi(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 1 || j < 1 || i > s.N || j > s.M) return;
+    const int W = s.P;
+    const double *h = s.h, *z = s.z;
+    s.Htdu[i * W + j] = (h[i * W + j - 1] + h[i * W + j] + z[(i + 1) * W + j] + z[i * W + j]) * 0.5;
+    s.Htdv[i * W + j] = (h[(i - 1) * W + j] + h[i * W + j] + z[i * W + j + 1] + z[i * W + j]) * 0.5;
+}
+
+// ---------------------------------------------------------------------------
+// preprocess_data (support_funcs.cpp:482-601): Q-boundary weights and
+// hmax_*.  One workgroup; one thread per boundary segment sums its
+// h^(5/3) terms in the reference's order.
+// ---------------------------------------------------------------------------
+__global__ void k_preprocess(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.x];
+    const int M = s.M, N = s.N, W = s.P;
+    const double ND = s.c.NANGDAY, dX = s.c.dX, ros = s.c.ros;
+    const double *h = s.h;
+    for (int side = 0; side < 4; side++) {
+        const int *nseg, *fs, *ls;
+        double *hs;
+        if (side == 0) {
+            nseg = s.segV_n + M; fs = s.segV_f + (size_t)M * s.capV; ls = s.segV_l + (size_t)M * s.capV;
+            hs = s.hi;
+        } else if (side == 1) {
+            nseg = s.segV_n + 2; fs = s.segV_f + (size_t)2 * s.capV; ls = s.segV_l + (size_t)2 * s.capV;
+            hs = s.hi + (N + 3);
+        } else if (side == 2) {
+            nseg = s.segH_n + 2; fs = s.segH_f + (size_t)2 * s.capH; ls = s.segH_l + (size_t)2 * s.capH;
+            hs = s.hi + 2 * (N + 3);
+        } else {
+            nseg = s.segH_n + N; fs = s.segH_f + (size_t)N * s.capH; ls = s.segH_l + (size_t)N * s.capH;
+            hs = s.hi + 2 * (N + 3) + (M + 3);
+        }
+        for (int k = threadIdx.x; k < *nseg; k += blockDim.x) {
+            double sum = 0;
+            for (int i = fs[k]; i <= ls[k]; i++) {
+                double d;
+                if (side == 0) d = h[i * W + M] + h[(i - 1) * W + M];
+                else if (side == 1) d = h[i * W + 1] + h[(i - 1) * W + 1];
+                else if (side == 2) d = h[1 * W + i] + h[1 * W + i - 1];
+                else d = h[N * W + i] + h[N * W + i - 1];
+                hs[i] = d / 2.0 - ND;
+                sum += pow(hs[i], 5.0 / 3.0);
+            }
+            for (int i = fs[k]; i <= ls[k]; i++) hs[i] = pow(hs[i], 2.0 / 3.0) / (sum * dX);
+        }
+    }
+    if (threadIdx.x == 0) {
+        const double *hb = s.htaiz_bd;
+        double hu = hb[2 * W + M], hd = hb[2 * W + 2];
+        for (int i = 3; i < N; i++) {
+            if (hb[i * W + M] > hu) hu = hb[i * W + M];
+            if (hb[i * W + 2] > hd) hd = hb[i * W + 2];
+        }
+        double hl = hb[2 * W + 2], hr = hb[N * W + 2];
+        for (int i = 3; i < M; i++) {
+            if (hb[2 * W + i] > hl) hl = hb[2 * W + i];
+            if (hb[N * W + i] > hr) hr = hb[N * W + i];
+        }
+        if (hu != 0) hu = 1.0 / (ros * hu);
+        if (hd != 0) hd = 1.0 / (ros * hd);
+        if (hl != 0) hl = 1.0 / (ros * hl);
+        if (hr != 0) hr = 1.0 / (ros * hr);
+        s.hmax[0] = hu;
+        s.hmax[1] = hd;
+        s.hmax[2] = hl;
+        s.hmax[3] = hr;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary update (Update_Boundary_Value, support_funcs.cpp:682-728).
+// One workgroup per scenario; sides in the order up, down, left, right with
+// a barrier between them so a later side wins at shared corner cells.
+// Only segment 0 of the boundary line receives a value (:614-618).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ S, double t_in) {
+    const DevState &s = S[blockIdx.x];
+    const int M = s.M, N = s.N, W = s.P;
+    const double t = (double)(float)t_in;  // engine.cpp:22 passes t as float
+    const int t1 = (int)(t / 3600);
+    const double t2 = (t - (3600.0f * t1)) / 3600.0f;
+    const double t2fs = (t - (3600.0 * t1)) / 3600.0;
+    const int k0 = min(t1, s.total_time - 1), k1 = min(t1 + 1, s.total_time - 1);
+    const int imax = max(M, N);
+    for (int side = 0; side < 4; side++) {
+        int n, f0, l0;
+        const double *hs;
+        double *barr;
+        bool isU;
+        int loc, ext;
+        if (side == 0) {
+            n = s.segV_n[M]; f0 = s.segV_f[(size_t)M * s.capV]; l0 = s.segV_l[(size_t)M * s.capV];
+            hs = s.hi; barr = s.vbt; isU = false; loc = M; ext = M + 1;
+        } else if (side == 1) {
+            n = s.segV_n[2]; f0 = s.segV_f[(size_t)2 * s.capV]; l0 = s.segV_l[(size_t)2 * s.capV];
+            hs = s.hi + (N + 3); barr = s.vbd; isU = false; loc = 2; ext = 1;
+        } else if (side == 2) {
+            n = s.segH_n[2]; f0 = s.segH_f[(size_t)2 * s.capH]; l0 = s.segH_l[(size_t)2 * s.capH];
+            hs = s.hi + 2 * (N + 3); barr = s.ubt; isU = true; loc = 2; ext = 1;
+        } else {
+            n = s.segH_n[N]; f0 = s.segH_f[(size_t)N * s.capH]; l0 = s.segH_l[(size_t)N * s.capH];
+            hs = s.hi + 2 * (N + 3) + (M + 3); barr = s.ubp; isU = true; loc = N; ext = N + 1;
+        }
+        if (n > 0) {
+            const double *bc = s.bc[side];
+            const double value = bc[k0] * (1.0f - t2) + bc[k1] * t2;
+            double fsval = 0.0;
+            if (s.hasCC[side]) fsval = s.CC[side][k0] * (1.0 - t2fs) + s.CC[side][k1] * t2fs;
+            const bool isQ = s.btype[side] == 2;
+            for (int i = 2 + threadIdx.x; i <= imax; i += blockDim.x) {
+                if (!(f0 <= i && i <= l0)) continue;
+                if (isQ) {
+                    barr[i] = value * hs[i];
+                    const int fi = isU ? loc : i, fj = isU ? i : loc;
+                    s.FS[fi * W + fj] = fsval * s.htaiz_bd[fi * W + fj] * s.hmax[side];
+                } else if (isU) {
+                    s.t_z[loc * W + i] = value;
+                    s.t_z[ext * W + i] = value;
+                } else {
+                    s.t_z[i * W + loc] = value;
+                    s.t_z[i * W + ext] = value;
+                }
+            }
+            if (isQ && threadIdx.x == 0) {
+                // prescribed discharge volume of this half-step (up/right sides face outward)
+                const double sign = (side == 0 || side == 3) ? -1.0 : 1.0;
+                s.acc[0] += sign * value * (0.5 * s.c.dT);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 implicit sweep: the interleaved (z, vel) system of every wet
+// segment, assembled and solved by Thomas in the reference's row order
+// (UVZSolver_multithread.cpp:19-43, :110-139, :280-603).
+//
+// One lane per line; the lanes of a wavefront walk the line positions in
+// lockstep, so DIR == 1 (UZ: line = column j, walk along i) reads each row
+// of the state coalesced.  Forward pass: coefficients + elimination, Ap/Bp
+// for two rows per cell in scratch laid out [2*p + slot][line].  Backward
+// pass: back-substitution and extraction into t_z and t_u (UZ) / t_v (VZ).
+// ---------------------------------------------------------------------------
+enum { END_WALL = 0, END_Q = 1, END_LONG = 2 };
+
+struct SegInfo {
+    int f, l, sn;
+    bool ran1, ran2, isB;
+    int mat_end;  // END_* used by the assembly
+};
+
+template <int DIR>
+HD_DEV SegInfo seg_info(const DevState &s, int line, int f, int l, bool dkQ1, bool dkQ2, bool q11) {
+    SegInfo g;
+    g.f = f;
+    g.l = l;
+    const int L = DIR == 1 ? s.N : s.M;
+    wall_flags(s, DIR, line, f, l, g.ran1, g.ran2);
+    int sn = 2 * (l - f);
+    if (g.ran1) {
+        g.isB = true;
+        if (!g.ran2) {
+            if (dkQ2 && l == L) {
+                g.mat_end = END_Q;
+            } else {
+                sn--;
+                g.mat_end = END_LONG;
+            }
+        } else {
+            g.mat_end = END_WALL;
+        }
+    } else {
+        g.isB = dkQ1 && f == 2;
+        const bool endQ = q11 ? dkQ1 : dkQ2;
+        if (!g.ran2) {
+            if (endQ && l == L) {
+                if (!g.isB) sn--;
+                g.mat_end = END_Q;
+            } else {
+                sn--;
+                if (!g.isB) sn -= 1;
+                g.mat_end = END_LONG;
+            }
+        } else {
+            if (!g.isB) sn--;
+            g.mat_end = END_WALL;
+        }
+    }
+    g.sn = sn;
+    return g;
+}
+
+template <int DIR>
+HD_DEV Pre preindex(const DevState &s, int line, int p) {
+    return DIR == 1 ? uz_preindex(s, p, line) : vz_preindex(s, line, p);
+}
+
+template <int DIR>
+__global__ void __launch_bounds__(64) k_sweep_plain(const DevState *__restrict__ S, int start, int end) {
+    const DevState &s = S[blockIdx.y];
+    const int line = start + blockIdx.x * 64 + threadIdx.x;
+    const bool active = line <= end;
+    const int L = DIR == 1 ? s.N : s.M;
+    const int W = s.P;
+    const int ld = DIR == 1 ? s.P : s.ldT;
+    const int *seg_n = DIR == 1 ? s.segV_n : s.segH_n;
+    const int cap = DIR == 1 ? s.capV : s.capH;
+    const int *segf = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)(active ? line : 0) * cap;
+    const int *segl = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)(active ? line : 0) * cap;
+    const int nseg = active ? seg_n[line] : 0;
+    const double f4 = DIR == 1 ? 2 * s.c.g * s.c.dTchia2dX : 2 * s.c.g * s.c.dTchia2dY;
+    const bool dkQ1 = DIR == 1 ? s.bienQ[2] : s.bienQ[1];
+    const bool dkQ2 = DIR == 1 ? s.bienQ[3] : s.bienQ[0];
+    const bool q11 = (s.quirks & HD_QUIRK_Q11_LITERAL) != 0;
+    const double ub_start = active ? (DIR == 1 ? s.ubt[line] : s.vbd[line]) : 0.0;
+    const double ub_end = active ? (DIR == 1 ? s.ubp[line] : s.vbt[line]) : 0.0;
+    double *__restrict__ Ap = s.Ap;
+    double *__restrict__ Bp = s.Bp;
+    double *__restrict__ tz = s.t_z;
+    double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
+    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
+    auto sidx = [&](int p, int slot) { return (size_t)(2 * p + slot) * ld + line; };
+
+    // ---------------- forward ----------------
+    {
+        int k = 0;
+        SegInfo g;
+        g.f = nseg > 0 ? segf[0] : L + 1;
+        g.l = nseg > 0 ? segl[0] : L;
+        g.sn = -1;
+        g.isB = false;
+        g.ran1 = g.ran2 = false;
+        g.mat_end = END_WALL;
+        double TZ_l = 0, TZ_r = 0, ApP = 0, BpP = 0;
+        Pre cur = {}, nxt = {};
+        for (int p = 2; p <= L; p++) {
+            if (k < nseg && p > g.l) {
+                k++;
+                if (k < nseg) {
+                    g.f = segf[k];
+                    g.l = segl[k];
+                }
+            }
+            if (!(k < nseg && p >= g.f && p <= g.l)) continue;
+            const int f = g.f, l = g.l;
+            if (p == f) {
+                g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
+                TZ_l = tz[cell(f)];
+                TZ_r = tz[cell(l)];
+                cur = preindex<DIR>(s, line, f);
+            }
+            const int sn = g.sn;
+            if (p + 1 <= l) nxt = preindex<DIR>(s, line, p + 1);
+            const int r0 = 2 * (p - f);
+            if (r0 > sn) {
+                cur = nxt;
+                continue;
+            }
+            const bool need_mom = p <= l - 1;
+            Mom m = {0, 0, 0, 0};
+            if (need_mom) m = calc_abcd(p, f, l, f4, g.ran1, g.ran2, cur, nxt);
+            double A[2], B[2], C[2], D[2];
+            if (g.isB) {
+                A[0] = cur.a2; B[0] = 1; C[0] = cur.c2; D[0] = cur.d2;
+                if (p == f && !g.ran1) D[0] = cur.d2 - cur.a2 * ub_start;
+                A[1] = m.a1; B[1] = m.b1; C[1] = m.c1; D[1] = m.d1;
+            } else {
+                A[0] = m.a1; B[0] = m.b1; C[0] = m.c1; D[0] = m.d1;
+                if (p == f) D[0] = m.d1 - m.a1 * TZ_l;
+                A[1] = nxt.a2; B[1] = 1; C[1] = nxt.c2; D[1] = nxt.d2;
+            }
+            for (int slot = 0; slot < 2; slot++) {
+                const int r = r0 + slot;
+                if (r > sn) break;
+                double a = A[slot], b = B[slot], cc = C[slot], d = D[slot];
+                if (r == sn) {
+                    // end row (the last write wins over the per-cell row)
+                    if (g.mat_end == END_WALL) {
+                        const Pre &e = g.isB ? cur : nxt;  // continuity of cell l
+                        a = e.a2; b = 1; d = e.d2;
+                    } else if (g.mat_end == END_Q) {
+                        const Pre &e = g.isB ? cur : nxt;
+                        a = e.a2; b = 1; d = e.d2 - e.c2 * ub_end;
+                    } else {
+                        a = m.a1; b = m.b1; d = m.d1 - m.c1 * TZ_r;
+                    }
+                }
+                double ap, bp;
+                if (r == 0) {
+                    if (sn == 0) {
+                        ap = 0;
+                        bp = d / b;
+                    } else {
+                        ap = -cc / b;
+                        bp = d / b;
+                    }
+                } else if (r < sn) {
+                    const double ep = a * ApP + b;
+                    ap = -cc / ep;
+                    bp = (d - (a * BpP)) / ep;
+                } else {
+                    ap = 0;
+                    bp = (d - (a * BpP)) / (b + (a * ApP));
+                }
+                Ap[sidx(p, slot)] = ap;
+                Bp[sidx(p, slot)] = bp;
+                ApP = ap;
+                BpP = bp;
+            }
+            cur = nxt;
+        }
+    }
+
+    // ---------------- backward ----------------
+    {
+        int k = nseg - 1;
+        SegInfo g;
+        g.f = nseg > 0 ? segf[k] : L + 1;
+        g.l = nseg > 0 ? segl[k] : L;
+        g.sn = -1;
+        double xnext = 0, x_sn = 0, tvel_lm1 = 0;
+        for (int p = L; p >= 2; p--) {
+            if (k >= 0 && p < g.f) {
+                k--;
+                if (k >= 0) {
+                    g.f = segf[k];
+                    g.l = segl[k];
+                }
+            }
+            if (!(k >= 0 && p >= g.f && p <= g.l)) continue;
+            const int f = g.f, l = g.l;
+            if (p == l) g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
+            const int sn = g.sn;
+            const int r0 = 2 * (p - f);
+            double x[2] = {0, 0};
+            for (int slot = 1; slot >= 0; slot--) {
+                const int r = r0 + slot;
+                if (r > sn) continue;
+                const double bp = Bp[sidx(p, slot)];
+                double xv;
+                if (r == sn) {
+                    xv = bp;
+                    x_sn = bp;
+                } else {
+                    xv = bp + (Ap[sidx(p, slot)] * xnext);
+                }
+                x[slot] = xv;
+                xnext = xv;
+            }
+            // interior cells (p < l)
+            if (p < l) {
+                if (g.isB) {
+                    tz[cell(p)] = x[0];
+                    tv[cell(p)] = x[1];
+                    if (p == l - 1) tvel_lm1 = x[1];
+                    if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : ub_start;
+                } else {
+                    tv[cell(p)] = x[0];
+                    if (p == l - 1) tvel_lm1 = x[0];
+                    if (p + 1 < l) tz[cell(p + 1)] = x[1];
+                    if (p == f) {
+                        const Pre e = preindex<DIR>(s, line, f);
+                        tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * x[0]) / e.a2;
+                    }
+                }
+            }
+            // end cell, once t_vel[l-1] and x[sn] are known
+            if (p == (l > f ? l - 1 : l)) {
+                const bool ext_Q = dkQ2 && l == L;
+                if (g.ran2) {
+                    tv[cell(l)] = 0;
+                    if (sn >= 0) tz[cell(l)] = x_sn;
+                } else if (ext_Q) {
+                    tv[cell(l)] = ub_end;
+                    if (sn >= 0) tz[cell(l)] = x_sn;
+                } else {
+                    const Pre e = preindex<DIR>(s, line, l);
+                    const double prev = l > f ? tvel_lm1 : tv[cell(l - 1)];
+                    tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
+                }
+            }
+        }
+    }
+}
+
+template __global__ void k_sweep_plain<0>(const DevState *, int, int);
+template __global__ void k_sweep_plain<1>(const DevState *, int, int);
+
+// ---------------------------------------------------------------------------
+// Smoothing (Normalize, support_funcs.cpp:761-808): tmp = t_vel everywhere,
+// Jacobi-smoothed on wet-interior cells [2,N] x [2,M].  The copy back into
+// t_vel (update_buffer) is done by the next kernel that touches each cell.
+// ---------------------------------------------------------------------------
+template <bool ISU>
+__global__ void k_smooth(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > s.N + 2 || j > s.M + 2) return;
+    const int W = s.P;
+    const double *__restrict__ val = ISU ? s.t_u : s.t_v;
+    const int closenb = ISU ? W : 1;
+    const int farnb = ISU ? 1 : W;
+    const int pos = i * W + j;
+    double out = val[pos];
+    if (i >= 2 && j >= 2 && i <= s.N && j <= s.M && out != 0) {
+        int count = 2;
+        double nb = val[pos - closenb] + val[pos + closenb];
+        if (s.khouot[pos - farnb] == 0) {
+            nb += val[pos - farnb];
+            count++;
+        }
+        if (s.khouot[pos + farnb] == 0) {
+            nb += val[pos + farnb];
+            count++;
+        }
+        out = out * s.c.heso + (1 - s.c.heso) * nb / count;
+    }
+    s.tmp[pos] = out;
+}
+template __global__ void k_smooth<true>(const DevState *);
+template __global__ void k_smooth<false>(const DevState *);
+
+HD_DEV bool interior(const DevState &s, int i, int j) { return i >= 2 && j >= 2 && i <= s.N && j <= s.M; }
+
+// Whether position p of a line (khouot values `kh` at stride `st`) lies
+// inside a segment and p+1 in the same segment: the range [first, last-1]
+// that solveV / solveU cover (UVZSolver_multithread.cpp:955,974).
+HD_DEV bool in_seg_not_last(const int *__restrict__ kh, int st, int p, int end) {
+    if (p < 2 || p > end - 1) return false;
+    if (kh[p * st] != 0 || kh[(p + 1) * st] != 0) return false;
+    // the run through p must start at end-2 or earlier
+    if (p <= end - 2) return true;
+    return kh[(end - 2) * st] == 0;  // p == end-1
+}
+
+// solveV (UVZSolver_multithread.cpp:961-978) on every row segment cell
+// j in [first, last-1], plus update_buffer of the smoothed t_u.
+__global__ void k_solve_v(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (!interior(s, i, j)) return;
+    const int W = s.P, pos = i * W + j;
+    const double *tu_s = s.tmp;  // smoothed t_u
+    if (in_seg_not_last(s.khouot + i * W, 1, j, s.M)) s.t_v[pos] = v_solver(s, tu_s, i, j);
+    s.t_u[pos] = tu_s[pos];
+}
+
+// solveU (UVZSolver_multithread.cpp:938-959) on every column segment cell
+// i in [first, last-1], plus update_buffer of the smoothed t_v.
+__global__ void k_solve_u(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (!interior(s, i, j)) return;
+    const int W = s.P, pos = i * W + j;
+    const double *tv_s = s.tmp;  // smoothed t_v
+    if (in_seg_not_last(s.khouot + j, W, i, s.N)) s.t_u[pos] = u_solver(s, tv_s, i, j);
+    s.t_v[pos] = tv_s[pos];
+}
+
+// update_margin_elem_V / _U (UVZSolver_multithread.cpp:980-1060): one
+// thread per line walks its segments in order.
+template <int DIR>
+__global__ void k_margin(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.y];
+    const int line = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int end = DIR == 1 ? s.M : s.N;  // lines: columns (U) or rows (V)
+    if (line > end) return;
+    const int W = s.P;
+    const double ND = s.c.NANGDAY;
+    const double *h = s.h;
+    const int n = DIR == 1 ? s.segV_n[line] : s.segH_n[line];
+    const int cap = DIR == 1 ? s.capV : s.capH;
+    const int *fs = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)line * cap;
+    const int *ls = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)line * cap;
+    double *tv = DIR == 1 ? s.t_u : s.t_v;
+    for (int k = 0; k < n; k++) {
+        const int first = fs[k], last = ls[k];
+        bool ran1 = false, ran2 = false;
+        if (DIR == 1) {
+            const int j = line, N = s.N;
+            if ((first > 2) || ((first == 2) && ((fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - ND) < 1e-16)))
+                ran1 = true;
+            if ((last < N) || ((last == N) && (fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - ND) < 1e-16)))
+                ran2 = true;
+            if (ran1)
+                tv[(first - 1) * W + j] = 0;
+            else
+                tv[(first - 1) * W + j] = 2 * tv[first * W + j] - tv[(first + 1) * W + j];
+            if (ran2)
+                tv[last * W + j] = 0;
+            else
+                tv[last * W + j] = 2 * tv[(last - 1) * W + j] - tv[(last - 2) * W + j];
+        } else {
+            const int i = line, M = s.M;
+            if ((first > 2) ||
+                ((first == 2) && (fabs((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 - ND) < 1e-16)))
+                ran1 = true;
+            if ((last < M) ||
+                ((last == M) && (fabs((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 - ND) < 1e-16)))
+                ran2 = true;
+            if (ran1)
+                tv[i * W + first - 1] = 0;
+            else
+                tv[i * W + first - 1] = 2 * tv[i * W + first] - tv[i * W + first + 1];
+            if (ran2)
+                tv[i * W + last] = 0;
+            else
+                tv[i * W + last] = 2 * tv[i * W + last - 1] - tv[i * W + last - 2];
+        }
+    }
+}
+template __global__ void k_margin<0>(const DevState *);
+template __global__ void k_margin<1>(const DevState *);
+
+// New water level at a cell after update_uvz: t_z inside [0,N]x[0,M],
+// the old z elsewhere (update_uvz, support_funcs.cpp:750).
+HD_DEV double z_after(const DevState &s, int i, int j) {
+    const int pos = i * s.P + j;
+    return (i <= s.N && j <= s.M) ? s.t_z[pos] : s.z[pos];
+}
+
+// End of the first half-step (engine.cpp:122-139): update_buffer(v),
+// update_h_moi, update_uvz and Htuongdoi in one pass.
+__global__ void k_post_a(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > s.N + 2 || j > s.M + 2) return;
+    const int W = s.P, pos = i * W + j;
+    if (interior(s, i, j)) s.t_v[pos] = s.tmp[pos];
+    if (i >= 1 && j >= 1 && i <= s.N && j <= s.M && s.khouot[pos] == 0)
+        s.H_moi[pos] = s.htaiz[pos] + s.t_z[pos];
+    const int kd = (int)s.c.kenhhepd, kn = (int)s.c.kenhhepng;
+    if (i <= s.N && j <= s.M) {
+        s.u[pos] = s.t_u[pos] * (1 - kd);
+        s.v[pos] = (interior(s, i, j) ? s.tmp[pos] : s.t_v[pos]) * (1 - kn);
+    }
+    if (i >= 1 && j >= 1 && i <= s.N && j <= s.M) {
+        const double *h = s.h;
+        const double zij = s.t_z[pos];
+        s.Htdu[pos] = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
+        s.Htdv[pos] = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+    }
+    // z last: other threads read z only outside [0,N]x[0,M], which is not written
+    if (i <= s.N && j <= s.M) s.z[pos] = s.t_z[pos];
+}
+
+// Second half-step before Reset_states_vertical (engine.cpp:229-236):
+// update_buffer(u) and update_h_moi.
+__global__ void k_pre_reset(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > s.N || j > s.M) return;
+    const int W = s.P, pos = i * W + j;
+    if (interior(s, i, j)) s.t_u[pos] = s.tmp[pos];
+    if (i >= 1 && j >= 1 && s.khouot[pos] == 0) s.H_moi[pos] = s.htaiz[pos] + s.t_z[pos];
+}
+
+// Reset_states_vertical (support_funcs.cpp:142-188), re-wetting pass: one
+// lane per column j in [1, M], serial along i (lanes read adjacent columns).
+__global__ void k_rewet(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.y];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    if (j > s.M) return;
+    const int W = s.P, N = s.N;
+    double *t_z = s.t_z, *H_moi = s.H_moi, *FS = s.FS;
+    const double *z = s.z, *htaiz = s.htaiz;
+    int *kh = s.khouot;
+    for (int i = 2; i <= N; i++) {
+        const double tzi = t_z[i * W + j];
+        if (tzi > z[i * W + j]) {
+            if (kh[(i - 1) * W + j] == 1) {
+                t_z[(i - 1) * W + j] = tzi;
+                H_moi[(i - 1) * W + j] = htaiz[(i - 1) * W + j] + tzi;
+                kh[(i - 1) * W + j] = 0;
+                FS[(i - 1) * W + j] = FS[i * W + j];
+            }
+            if (kh[(i + 1) * W + j] == 1) {
+                t_z[(i + 1) * W + j] = tzi;
+                H_moi[(i + 1) * W + j] = htaiz[(i + 1) * W + j] + tzi;
+                kh[(i + 1) * W + j] = 0;
+                FS[(i + 1) * W + j] = FS[i * W + j];
+            }
+        }
+    }
+}
+
+// Reset_states_vertical, drying pass: independent per cell once re-wetting
+// is done (writes of t_u/t_v are zeros, so shared faces agree).
+__global__ void k_dry(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 2 || j < 1 || i > s.N || j > s.M) return;
+    const int W = s.P, pos = i * W + j;
+    if ((s.khouot[pos] == 0) && (s.H_moi[pos] <= s.c.H_TINH)) {
+        s.t_u[pos - W] = 0;
+        s.t_u[pos] = 0;
+        s.t_v[pos - 1] = 0;
+        s.t_v[pos] = 0;
+        s.khouot[pos] = 1;
+        s.FS[pos] = 0;
+    }
+}
+
+// End of the second half-step (engine.cpp:247-255): update_uvz and Htuongdoi.
+__global__ void k_post_b(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i > s.N || j > s.M) return;
+    const int W = s.P, pos = i * W + j;
+    const int kd = (int)s.c.kenhhepd, kn = (int)s.c.kenhhepng;
+    s.u[pos] = s.t_u[pos] * (1 - kd);
+    s.v[pos] = s.t_v[pos] * (1 - kn);
+    if (i >= 1 && j >= 1) {
+        const double *h = s.h;
+        const double zij = s.t_z[pos];
+        s.Htdu[pos] = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
+        s.Htdv[pos] = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+    }
+    s.z[pos] = s.t_z[pos];
+}
+
+// Interpolate_FS_ng on rows N and 2, then Interpolate_FS_d on column M
+// (support_funcs.cpp:191-237, called from the segmentation kernels).
+__global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.x];
+    const int M = s.M, N = s.N, W = s.P;
+    const double dT = s.c.dT, dX = s.c.dX, dY = s.c.dY;
+    double *FS = s.FS;
+    for (int pass = 0; pass < 2; pass++) {
+        const int location = pass == 0 ? N : 2, offset = pass == 0 ? N - 1 : 2, sign = pass == 0 ? -1 : 1;
+        for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
+            if (s.t_u[location * W + j] > 0) {
+                double deltaX = (s.u[location * W + j] + s.t_u[location * W + j]) * 0.25 * dT;
+                int i = (int)(offset + sign * (deltaX / dX));
+                int i_deltaX = __double2int_rn(deltaX);
+                int i_dX = __double2int_rn(dX);
+                if (i < 0 || i + 1 > N + 2) continue;
+                FS[location * W + j] =
+                    FS[i * W + j] + sign * (FS[(i + 1) * W + j] - FS[i * W + j]) * (deltaX / dX - i_deltaX / i_dX);
+            }
+        }
+        __syncthreads();
+    }
+    const int iend = min(M - 1, N + 2);
+    const int location = M, offset = M - 1, sign = -1;
+    for (int i = 1 + threadIdx.x; i <= iend; i += blockDim.x) {
+        if (s.t_v[i * W + location] > 0) {
+            double deltaY = (s.v[i * W + location] + s.t_v[i * W + location]) * 0.25 * dT;
+            int j = (int)(offset + sign * (deltaY / dY));
+            int j_deltaY = __double2int_rn(deltaY);
+            int j_dY = __double2int_rn(dY);
+            if (j < 0 || j + 1 > M + 2) continue;
+            FS[i * W + location] = FS[i * W + j + 1] + (FS[i * W + j + 1] - FS[i * W + j]) * (deltaY / dY - j_deltaY / j_dY);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sediment
+// ---------------------------------------------------------------------------
+
+// Find_VTH (sediment_transport.cpp:575-602) and hesoK (:539-573).
+__global__ void k_vth_k(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 1 || j < 1 || i > s.N || j > s.M) return;
+    const int W = s.P, pos = i * W + j;
+    if (i >= 2 && j >= 2) {
+        if ((s.htaiz[pos] > s.c.NANGDAY) && (s.H_moi[pos] > s.c.H_TINH)) {
+            double ut = (s.t_u[pos] + s.t_u[pos - W]) * 0.5;
+            double vt = (s.t_v[pos] + s.t_v[pos - 1]) * 0.5;
+            s.VTH[pos] = sqrt(ut * ut + vt * vt);
+        } else
+            s.VTH[pos] = 0;
+    }
+    const double Ks = s.c.Ks, g = s.c.g;
+    if (s.Htdu[pos] > 0) {
+        double Cz = 18 * log(12 * s.Htdu[pos] / Ks);
+        double k = 5.93 * sqrt(g) * (s.h[pos - 1] + s.h[pos]) * 0.25 * fabs(s.t_u[pos]) / Cz;
+        s.Kx[pos] = fmin(100.0, fmax(5.0, k));
+    }
+    if (s.Htdv[pos] > 0) {
+        double Cz = 18 * log(12 * s.Htdv[pos] / Ks);
+        double k = 5.93 * sqrt(g) * (s.h[pos - W] + s.h[pos]) * 0.25 * fabs(s.t_v[pos]) / Cz;
+        s.Ky[pos] = fmin(100.0, fmax(5.0, k));
+    }
+}
+
+// FS row of cell (i, j) of a column segment [first, last]
+// (_FSj_calculate__mactrix_coeff, sediment_transport.cpp:204-313).
+HD_DEV void fs_row_j(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    const double gamav = 0.98 - 0.198 * 0.0 + 0.032 * 0.0 * 0.0;  // formed before Zf (:247)
+    FSCell fc = fs_cell(s, pos, t, s_start);
+    A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos - W] * Kx[pos - W] / (H[pos] * (dX * dX));
+    Cc = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos] * Kx[pos] / (H[pos] * (dX * dX));
+    B = 2.0 / dT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (H[pos] * (dX * dX));
+    double p = 0, q = 0;
+    if ((H[pos - 1] > H_TINH) && (H[pos + 2] > H_TINH)) p = (FS[pos + 1] - FS[pos]);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 1] > H_TINH)) q = (FS[pos] - FS[pos - 1]);
+    p = (1 / (H[pos] * (dY * dY))) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 2] > H_TINH)) {
+        q = (FS[pos + 1] - FS[pos - 1]) / (dY * 2);
+        q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
+    } else
+        q = 0;
+    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
+    if (i == first + 1) {
+        if ((ran1) || (t_u[first * W + j] == 0))
+            B += A;
+        else
+            D -= A * FS[first * W + j];
+    }
+    if (i == last - 1) {
+        if ((ran2) && (t_u[last * W + j] == 0))
+            B += Cc;
+        else
+            D -= Cc * FS[last * W + j];
+    }
+}
+
+// FS row of cell (i, j) of a row segment
+// (_FSi_calculate__mactrix_coeff, sediment_transport.cpp:51-164).
+HD_DEV void fs_row_i(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    FSCell fc = fs_cell(s, pos, t, s_start);
+    const double gamav = 0.98 - 0.198 * fc.Zf + 0.032 * fc.Zf * fc.Zf;
+    A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos - 1] * Ky[pos - 1] / (H[pos] * (dY * dY));
+    Cc = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos] * Ky[pos] / (H[pos] * (dY * dY));
+    B = 2.0 / dT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (H[pos] * (dY * dY));
+    double p = 0, q = 0;
+    if ((H[pos + 2 * W] > H_TINH) && (H[pos - W] > H_TINH)) p = (FS[pos + W] - FS[pos]);
+    if (H[pos - 2 * W] > H_TINH && H[pos + W] > H_TINH) q = (FS[pos] - FS[pos - W]);
+    p = (1 / (H[pos] * (dX * dX))) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+    if (H[pos + 2 * W] > H_TINH && H[pos - 2 * W] > H_TINH) {
+        q = (FS[pos + W] - FS[pos - W]) / (dX * 2);
+        q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
+    } else
+        q = 0;
+    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
+    if (j == first + 1) {
+        if ((ran1) || (t_v[i * W + first] == 0))
+            B = B + A;
+        else
+            D = D - A * FS[i * W + first];
+    }
+    if (j == last - 1) {
+        if ((ran2) || (t_v[i * W + last] == 0))
+            B = B + Cc;
+        else
+            D = D - Cc * FS[i * W + last];
+    }
+}
+
+// FS sweep: Thomas per segment of at least three cells, one lane per line
+// (DIR 1: columns j in [start, end], along i; DIR 0: rows, along j), with
+// the extraction of _FSj/_FSi_extract_solution (:316-349, :167-201) into tFS.
+template <int DIR>
+__global__ void __launch_bounds__(64) k_fs_sweep(const DevState *__restrict__ S, int start, int end, double t,
+                                                  double s_start) {
+    const DevState &s = S[blockIdx.y];
+    const int line = start + blockIdx.x * 64 + threadIdx.x;
+    const bool active = line <= end;
+    const int L = DIR == 1 ? s.N : s.M;
+    const int W = s.P;
+    const int ld = DIR == 1 ? s.P : s.ldT;
+    const int cap = DIR == 1 ? s.capV : s.capH;
+    const int *segf = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)(active ? line : 0) * cap;
+    const int *segl = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)(active ? line : 0) * cap;
+    const int nseg = active ? (DIR == 1 ? s.segV_n : s.segH_n)[line] : 0;
+    double *__restrict__ Ap = s.Ap;
+    double *__restrict__ Bp = s.Bp;
+    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
+    auto sidx = [&](int p) { return (size_t)p * ld + line; };
+    // forward
+    {
+        int k = 0, f = nseg > 0 ? segf[0] : L + 1, l = nseg > 0 ? segl[0] : L;
+        bool ran1 = false, ran2 = false;
+        double ApP = 0, BpP = 0;
+        for (int p = 2; p <= L; p++) {
+            if (k < nseg && p > l) {
+                k++;
+                if (k < nseg) {
+                    f = segf[k];
+                    l = segl[k];
+                }
+            }
+            if (!(k < nseg && p >= f + 1 && p <= l - 1)) continue;
+            if (p == f + 1) wall_flags(s, DIR, line, f, l, ran1, ran2);
+            const int sn = l - f - 2, r = p - f - 1;
+            double a, b, cc, d;
+            if (DIR == 1)
+                fs_row_j(s, p, line, f, l, ran1, ran2, t, s_start, a, b, cc, d);
+            else
+                fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, a, b, cc, d);
+            double ap, bp;
+            if (r == 0) {
+                if (sn == 0) {
+                    ap = 0;
+                    bp = d / b;
+                } else {
+                    ap = -cc / b;
+                    bp = d / b;
+                }
+            } else if (r < sn) {
+                const double ep = a * ApP + b;
+                ap = -cc / ep;
+                bp = (d - (a * BpP)) / ep;
+            } else {
+                ap = 0;
+                bp = (d - (a * BpP)) / (b + (a * ApP));
+            }
+            Ap[sidx(p)] = ap;
+            Bp[sidx(p)] = bp;
+            ApP = ap;
+            BpP = bp;
+        }
+    }
+    // backward + extraction
+    {
+        int k = nseg - 1, f = nseg > 0 ? segf[k] : L + 1, l = nseg > 0 ? segl[k] : L;
+        bool ran1 = false, ran2 = false;
+        double xnext = 0;
+        double *tFS = s.tFS;
+        const double *FS = s.FS;
+        const double *tvel = DIR == 1 ? s.t_u : s.t_v;
+        const double NDnen = s.c.NDnen;
+        for (int p = L; p >= 2; p--) {
+            if (k >= 0 && p < f) {
+                k--;
+                if (k >= 0) {
+                    f = segf[k];
+                    l = segl[k];
+                }
+            }
+            if (!(k >= 0 && p >= f + 1 && p <= l - 1)) continue;
+            if (p == l - 1) wall_flags(s, DIR, line, f, l, ran1, ran2);
+            const int sn = l - f - 2, r = p - f - 1;
+            const double bp = Bp[sidx(p)];
+            const double xv = (r == sn) ? bp : bp + (Ap[sidx(p)] * xnext);
+            xnext = xv;
+            const double out = xv < 0 ? NDnen : xv;
+            tFS[cell(p)] = out;
+            if (p == l - 1) {
+                if ((ran2) || (tvel[cell(l)] == 0))
+                    tFS[cell(l)] = out;
+                else
+                    tFS[cell(l)] = FS[cell(l)];
+            }
+            if (p == f + 1) {
+                if ((ran1) || (tvel[cell(f)] == 0))
+                    tFS[cell(f)] = out;
+                else
+                    tFS[cell(f)] = FS[cell(f)];
+            }
+        }
+    }
+}
+template __global__ void k_fs_sweep<0>(const DevState *, int, int, double, double);
+template __global__ void k_fs_sweep<1>(const DevState *, int, int, double, double);
+
+// Update_FS (sediment_transport.cpp:654-663).
+__global__ void k_update_fs(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 1 || j < 1 || i > s.N || j > s.M) return;
+    const int pos = i * s.P + j;
+    s.FS[pos] = s.tFS[pos];
+}
+
+// Calculate_Qb (sediment_transport.cpp:352-434), interior cells.  Q8: Qb is
+// formed where VTH > 0 on wet cells and Toe is the coefficient.
+__global__ void k_qb(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 2 || j < 2 || i > s.N || j > s.M) return;
+    const int W = s.P, pos = i * W + j;
+    const hd_coeffs &c = s.c;
+    double qx = 0, qy = 0;
+    double Toee = c.Toe, Tx = 0;
+    const double vth = s.VTH[pos];
+    if ((vth > 0) && (s.khouot[pos] == 0)) {
+        double Tob = c.ro * s.Fw[pos] * (vth * vth);
+        if (s.H_moi[pos] > c.ghtoe) Toee = c.Toe * (1 + c.hstoe * ((s.H_moi[pos] - c.ghtoe)));
+        if (Tob > Toee) Tx = Tob / Toee - 1;
+        qx = 0.053 * pow((c.Sx - 1) * c.g, 0.5) * pow(c.dm, 1.5) * pow(Tx, 2.1) * pow(c.Dxr, -0.3) *
+             (s.t_u[pos] + s.t_u[pos - W]) * 0.5 / vth;
+        qy = 0.053 * pow((c.Sx - 1) * c.g, 0.5) * pow(c.dm, 1.5) * pow(Tx, 2.1) * pow(c.Dxr, -0.3) *
+             (s.t_v[pos] + s.t_v[pos - 1]) * 0.5 / vth;
+    }
+    const int N = s.N, M = s.M;
+    double *Qbx = s.Qbx, *Qby = s.Qby;
+    Qbx[pos] = qx;
+    Qby[pos] = qy;
+    if (i == 2) { Qbx[pos - W] = qx; Qby[pos - W] = qy; }
+    if (i == N) { Qbx[pos + W] = qx; Qby[pos + W] = qy; }
+    if (j == 2) {
+        Qbx[pos - 1] = qx; Qby[pos - 1] = qy;
+        if (i == 2) { Qbx[1 * W + 1] = qx; Qby[1 * W + 1] = qy; }
+        if (i == N) { Qbx[(N + 1) * W + 1] = qx; Qby[(N + 1) * W + 1] = qy; }
+    }
+    if (j == M) {
+        Qbx[pos + 1] = qx; Qby[pos + 1] = qy;
+        if (i == 2) { Qbx[1 * W + M + 1] = qx; Qby[1 * W + M + 1] = qy; }
+        if (i == N) { Qbx[(N + 1) * W + M + 1] = qx; Qby[(N + 1) * W + M + 1] = qy; }
+    }
+}
+
+// BedLoad -> _bed_load (sediment_transport.cpp:436-537, :664-673) on rows
+// i in [3, N-2] and row segments of at least three cells.  Q9: the downwind
+// test is t_u[pos] > 0.
+__global__ void k_bedload(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 3 || i > s.N - 2 || j < 2 || j > s.M) return;
+    const int W = s.P, pos = i * W + j, M = s.M, N = s.N;
+    const int n = s.segH_n[i];
+    const int *fs = s.segH_f + (size_t)i * s.capH, *ls = s.segH_l + (size_t)i * s.capH;
+    const int k = find_segment(fs, ls, n, j);
+    if (k < 0) return;
+    const int first = fs[k], last = ls[k];
+    if (last - first < 2) return;
+    const hd_coeffs &c = s.c;
+    const double dX = c.dX, dY = c.dY, dT = c.dT;
+    const double *Qbx = s.Qbx, *Qby = s.Qby, *FS = s.FS;
+    const int *kh = s.khouot;
+    double p = 0, q = 0;
+    if (kh[pos - W] == 1) {
+        if (s.t_u[pos] < 0) p = (-3 * Qbx[pos] + 4 * Qbx[pos + W] - Qbx[pos + 2 * W]) / (dX * 2);
+    } else {
+        if (kh[pos + W] == 1) {
+            if (s.t_u[pos] > 0) p = (3 * Qbx[pos] - 4 * Qbx[pos - W] + Qbx[pos - 2 * W]) / (dX * 2);
+        } else
+            p = (Qbx[pos + W] - Qbx[pos - W]) / (dX * 2);
+    }
+    if (kh[pos - 1] == 1) {
+        if (s.t_v[pos] < 0) q = (-3 * Qby[pos] + 4 * Qby[pos + 1] - Qby[pos + 2]) / (dY * 2);
+    } else {
+        if (kh[pos + 1] == 1) {
+            if (s.t_v[pos] > 0) q = (3 * Qby[pos] - 4 * Qby[pos - 1] + Qby[pos - 2]) / (dY * 2);
+        } else
+            q = (Qby[pos + 1] - Qby[pos - 1]) / (dY * 2);
+    }
+    double tH = p + q;
+    p = FS[pos + W] - FS[pos];
+    if ((kh[pos + 2 * W] == 1 || kh[pos - W] == 1) && (i < N)) p = 0;
+    q = FS[pos] - FS[pos - W];
+    if ((i > 2) && (kh[pos - 2 * W] == 1 || kh[pos + W] == 1)) q = 0;
+    p = 1 / (dX * dX) * (s.Htdu[pos] * s.Kx[pos] * p - s.Htdu[pos - W] * s.Kx[pos - W] * q);
+    tH = tH + p;
+    p = FS[pos + 1] - FS[pos];
+    if (((last < M) && (kh[pos - 1] == 1 || kh[pos + 2] == 1)) || (last == M && (kh[pos - 1] == 1))) p = 0;
+    q = FS[pos] - FS[pos - 1];
+    if (((first > 2) && (kh[pos - 2] == 1 || kh[pos + 1] == 1)) || (first == 2 && kh[pos + 1] == 1)) q = 0;
+    p = 1 / ((dY * dY)) * (s.Htdv[pos] * s.Ky[pos] * p - s.Htdv[pos - 1] * s.Ky[pos - 1] * q);
+    tH = tH + p;
+    const double H = s.H_moi[pos], vth = s.VTH[pos], fsv = FS[pos];
+    double cc = 18 * log(12 * H / c.Ks);
+    double Uf = sqrt(c.g) * fabs(vth) / cc;
+    double wsm = c.wss * pow((1 - fsv), 4);
+    double Zf = 0, Sv = 0;
+    if (fsv > 0.0) Zf = wsm / (0.4 * (Uf + 2 * wsm));
+    if (s.ketdinh)
+        Sv = source_chs(c, wsm, Zf, fsv, s.Fw[pos], vth, H);
+    else
+        Sv = source_nchs(c, wsm, Zf, Uf, H, fsv);
+    s.dH[pos] += dT / (1 - c.Dorong) * (Sv + tH);
+}
+
+// Domain volume over wet compute cells, (htaiz + z) * dX * dY: per-row
+// partial sums in column order, then rows in order (one workgroup).
+__global__ void __launch_bounds__(1024) k_volume(const DevState *__restrict__ S) {
+    const DevState &s = S[blockIdx.x];
+    double *rowsum = s.acc + 8;
+    const int W = s.P;
+    for (int i = 2 + threadIdx.x; i <= s.N; i += blockDim.x) {
+        double a = 0;
+        for (int j = 2; j <= s.M; j++) {
+            const int p = i * W + j;
+            if (s.khouot[p] == 0) a += (s.htaiz[p] + s.z[p]) * s.c.dX * s.c.dY;
+        }
+        rowsum[i] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0;
+        for (int i = 2; i <= s.N; i++) v += rowsum[i];
+        s.acc[1] = v;
+    }
+}
+
+// NaN/Inf watchdog over the water level: sets flag[0] = 1 when found.
+__global__ void k_watchdog(const DevState *__restrict__ S, int *flag) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 2 || j < 2 || i > s.N || j > s.M) return;
+    const int pos = i * s.P + j;
+    if (s.khouot[pos] == 0 && !isfinite(s.z[pos])) atomicOr(flag, 1);
+}
+
+}  // namespace hd

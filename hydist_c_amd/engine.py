@@ -1,0 +1,203 @@
+"""Python binding of the gfx950 library (libhydist_amd.so) over its C ABI.
+
+`Simulation` mirrors the reference's host driver: create (allocation and
+upload, loader.cpp:224-375), init (main.cpp:257-299), step / run
+(Hydraulic_Calculation, engine.cpp:41-296) and field access (save_result,
+loader.cpp:397-414).  There is no CPU path: loading fails loudly when the
+library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import F, MeshHolder, make_config, make_options
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libhydist_amd.so")
+
+_lib = None
+
+
+class HydistError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise HydistError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.hd_load_coeffs.argtypes = [C.POINTER(abi.Coeffs)]
+        L.hd_finalize_coeffs.argtypes = [C.POINTER(abi.Coeffs)]
+        L.hd_create.argtypes = [C.POINTER(abi.Mesh), C.POINTER(abi.Coeffs), C.POINTER(abi.Options),
+                                C.POINTER(abi.Config), C.POINTER(vp)]
+        for n in ("hd_init", "hd_sync", "hd_destroy"):
+            getattr(L, n).argtypes = [vp]
+        for n in ("hd_update_boundary", "hd_halfstep_uz", "hd_halfstep_vz"):
+            getattr(L, n).argtypes = [vp, C.c_double]
+        L.hd_step.argtypes = [vp, dp]
+        L.hd_run.argtypes = [vp, dp, C.c_double]
+        for n in ("hd_copy_out", "hd_copy_in", "hd_copy_out_boundary", "hd_copy_in_boundary"):
+            getattr(L, n).argtypes = [vp, C.c_int, dp, C.c_size_t]
+        for n in ("hd_copy_out_i32", "hd_copy_in_i32"):
+            getattr(L, n).argtypes = [vp, C.c_int, ip, C.c_size_t]
+        L.hd_get_hmax.argtypes = [vp, dp]
+        L.hd_set_hmax.argtypes = [vp, dp]
+        L.hd_get_segments.argtypes = [vp, C.c_int, ip, C.c_size_t, ip, ip, C.c_size_t, C.POINTER(C.c_int64)]
+        L.hd_mass_balance.argtypes = [vp, dp, dp]
+        L.hd_last_error.argtypes = [vp]
+        L.hd_last_error.restype = C.c_char_p
+        L.hd_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        _lib = L
+    return _lib
+
+
+def check(rc, handle=None):
+    if rc != 0:
+        msg = lib().hd_last_error(handle).decode(errors="replace")
+        raise HydistError(f"hydist error {rc}: {msg}")
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    rc = lib().hd_device_info(buf, 256)
+    return rc, buf.value.decode()
+
+
+def load_coeffs(overrides=None):
+    c = abi.Coeffs()
+    check(lib().hd_load_coeffs(C.byref(c)))
+    if overrides:
+        for k, v in overrides.items():
+            setattr(c, k, v)
+        check(lib().hd_finalize_coeffs(C.byref(c)))
+    return c
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Simulation:
+    """One scenario resident on the GPU."""
+
+    def __init__(self, scen, form=abi.HD_FORM_FP64_PLAIN, sediment=False, bed_change=False, quirks=0,
+                 Tmax=3600.0, coeffs=None, device=0):
+        self.scen = scen
+        self.shape = scen.shape
+        self.holder = MeshHolder(scen)
+        self.coeffs = coeffs if coeffs is not None else load_coeffs(scen.coeff_overrides)
+        self.ops = make_options(scen, Tmax)
+        self.cfg = make_config(form=form, sediment=sediment, bed_change=bed_change, quirks=quirks, device=device)
+        h = C.c_void_p()
+        rc = lib().hd_create(C.byref(self.holder.mesh), C.byref(self.coeffs), C.byref(self.ops),
+                             C.byref(self.cfg), C.byref(h))
+        check(rc)
+        self.h = h
+        self.t = 0.0
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            lib().hd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self):
+        check(lib().hd_init(self.h), self.h)
+
+    def step(self, n=1):
+        t = C.c_double(self.t)
+        for _ in range(n):
+            check(lib().hd_step(self.h, C.byref(t)), self.h)
+        self.t = t.value
+
+    def halfstep_uz(self, t):
+        check(lib().hd_halfstep_uz(self.h, t), self.h)
+
+    def halfstep_vz(self, t):
+        check(lib().hd_halfstep_vz(self.h, t), self.h)
+
+    def sync(self):
+        check(lib().hd_sync(self.h), self.h)
+
+    def get(self, name):
+        if name == "KHOUOT":
+            a = np.empty(self.shape, dtype=np.int32)
+            check(lib().hd_copy_out_i32(self.h, F[name], a.ctypes.data_as(C.POINTER(C.c_int32)), a.size), self.h)
+            return a
+        a = np.empty(self.shape, dtype=np.float64)
+        check(lib().hd_copy_out(self.h, F[name], _dp(a), a.size), self.h)
+        return a
+
+    def set(self, name, a):
+        if name == "KHOUOT":
+            a = np.ascontiguousarray(a, dtype=np.int32)
+            check(lib().hd_copy_in_i32(self.h, F[name], a.ctypes.data_as(C.POINTER(C.c_int32)), a.size), self.h)
+            return
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().hd_copy_in(self.h, F[name], _dp(a), a.size), self.h)
+
+    def boundary_size(self, which):
+        M, N = self.scen.M, self.scen.N
+        return {"UBT": M + 3, "UBP": M + 3, "VBT": N + 3, "VBD": N + 3, "HI": 2 * (M + N + 6)}[which]
+
+    def get_boundary(self, which):
+        a = np.empty(self.boundary_size(which))
+        check(lib().hd_copy_out_boundary(self.h, abi.BOUNDARY[which], _dp(a), a.size), self.h)
+        return a
+
+    def set_boundary(self, which, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().hd_copy_in_boundary(self.h, abi.BOUNDARY[which], _dp(a), a.size), self.h)
+
+    def get_hmax(self):
+        a = np.empty(4)
+        check(lib().hd_get_hmax(self.h, _dp(a)), self.h)
+        return a
+
+    def set_hmax(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().hd_set_hmax(self.h, _dp(a)), self.h)
+
+    def segments(self, direction):
+        nlines = self.shape[0] if direction == 0 else self.shape[1]
+        counts = np.zeros(nlines, dtype=np.int32)
+        total = C.c_int64()
+        ipc = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        check(lib().hd_get_segments(self.h, direction, ipc(counts), nlines, None, None, 0, C.byref(total)), self.h)
+        n = total.value
+        first = np.zeros(max(n, 1), dtype=np.int32)
+        last = np.zeros(max(n, 1), dtype=np.int32)
+        check(lib().hd_get_segments(self.h, direction, ipc(counts), nlines, ipc(first), ipc(last), n,
+                                    C.byref(total)), self.h)
+        return counts, first[:n], last[:n]
+
+    def mass_balance(self):
+        v, q = C.c_double(), C.c_double()
+        check(lib().hd_mass_balance(self.h, C.byref(v), C.byref(q)), self.h)
+        return v.value, q.value
+
+
+def _stage_api():
+    L = lib()
+    L.hd_set_profiling.argtypes = [C.c_void_p, C.c_int]
+    L.hd_stage_stats.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t, C.c_int]
+    return L
+
+
+def set_profiling(sim, on):
+    check(_stage_api().hd_set_profiling(sim.h, int(on)), sim.h)
+
+
+def stage_stats(sim, reset=True):
+    """{stage: (launches, total_ms)} measured with HIP events on the
+    simulation's stream."""
+    import json
+    buf = C.create_string_buffer(1 << 16)
+    check(_stage_api().hd_stage_stats(sim.h, buf, len(buf), int(reset)), sim.h)
+    return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}

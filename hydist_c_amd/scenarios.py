@@ -169,9 +169,9 @@ def meander_islands(M, N, seed=2, n_islands=30, amplitude=None, wavelength=None,
     shores; up side Q inflow with FS concentration U[0.05, 0.2] kg/m^3,
     down side water level."""
     rng = np.random.default_rng(seed)
-    amplitude = amplitude if amplitude is not None else 0.12 * N
-    wavelength = wavelength if wavelength is not None else 0.8 * M
-    half_width = half_width if half_width is not None else 0.30 * N
+    amplitude = amplitude if amplitude is not None else 0.25 * N
+    wavelength = wavelength if wavelength is not None else 0.5 * M
+    half_width = half_width if half_width is not None else 0.24 * N
     yc0 = N / 2.0
 
     def centre(x):

@@ -214,6 +214,13 @@ const char *hd_last_error(hd_ctx *ctx);
  * build flags) into buf. */
 int hd_device_info(char *buf, size_t n);
 
+/* Stage timing (no counterpart in the reference, which has no timers):
+ * when on, every kernel launch of the context is bracketed by HIP events
+ * on its stream.  hd_stage_stats writes {"stage": [launches, total_ms], ...}
+ * as JSON into buf (optionally clearing the counters). */
+int hd_set_profiling(hd_ctx *ctx, int on);
+int hd_stage_stats(hd_ctx *ctx, char *buf, size_t n, int reset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,134 @@
+"""GPU parity: the gfx950 plain FP64 form against the CPU oracle.
+
+Hydraulic fields are compared bit for bit (every value is formed from
++ - * / sqrt in the reference's order with contraction off).  Fields that
+involve pow/log (init Kx1/Ky1/hi, sediment FS/Kx/Ky/dH) are compared with
+stated tolerances; the pow() init fields are then copied into the oracle
+so the trajectories start from the same state."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from hydist_c_amd import scenarios as S
+from hydist_c_amd.abi import HD_QUIRK_Q11_LITERAL
+from parity_util import HYDRO, bits_equal, diff_report, make_pair, ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.mark.parametrize("maker", [lambda: S.open_box(40, 28), lambda: S.config1(),
+                                   lambda: S.meander_islands(96, 64, n_islands=6)])
+def test_init_matches_oracle(maker):
+    sc = maker()
+    _print_case("init", name=sc.name)
+    g, o = make_pair(sc, seed_init=False)
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for f in ("HTAIZ", "HTAIZ_BD", "HTDU", "HTDV", "HMOI"):
+        assert bits_equal(g.get(f), o.get(f)), f
+    # pow() on the device vs glibc: a few ulp
+    for f in ("KX1", "KY1"):
+        a, b = g.get(f), o.get(f)
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin), f
+        assert ulp_diff(a[fin], b[fin]).max() <= 4, f
+    hg, ho = g.get_boundary("HI"), o.get_boundary("HI")
+    assert np.allclose(hg, ho, rtol=1e-13, atol=0)
+    assert np.allclose(g.get_hmax(), o.get_hmax(), rtol=1e-15, atol=0)
+    for d in (0, 1):
+        cg, fg, lg = g.segments(d)
+        co, fo, lo = o.segments(d)
+        assert np.array_equal(cg, co) and np.array_equal(fg, fo) and np.array_equal(lg, lo)
+
+
+@pytest.mark.parametrize("maker,steps", [
+    (lambda: S.open_box(40, 28), 12),
+    (lambda: S.open_box(57, 33, seed=9, types=(0, 2, 0, 2)), 12),
+    (lambda: S.config1(), 30),
+    (lambda: S.meander_islands(96, 64, n_islands=6), 20),
+    (lambda: S.closed_basin(48, 36, z0_amp=0.3), 25),
+])
+def test_plain_fp64_bitwise(maker, steps):
+    sc = maker()
+    _print_case("plain_bitwise", name=sc.name, steps=steps)
+    g, o = make_pair(sc)
+    for k in range(steps):
+        g.step()
+        o.step()
+    rep = diff_report(g, o, HYDRO)
+    bad = {k: v for k, v in rep.items() if not v["bitwise"]}
+    assert not bad, bad
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for d in (0, 1):
+        assert all(np.array_equal(a, b) for a, b in zip(g.segments(d), o.segments(d)))
+
+
+def test_plain_fp64_quirk_q11_literal():
+    sc = S.open_box(40, 28, types=(2, 0, 0, 0))
+    _print_case("q11_literal", name=sc.name)
+    g, o = make_pair(sc, quirks=HD_QUIRK_Q11_LITERAL)
+    g.step(4)
+    o.step(4)
+    rep = diff_report(g, o, HYDRO)
+    assert all(v["bitwise"] for v in rep.values()), rep
+
+
+def test_half_steps_match_individually():
+    """Stage parity: after each half-step separately."""
+    sc = S.open_box(36, 26, seed=5)
+    _print_case("halfsteps", name=sc.name)
+    g, o = make_pair(sc)
+    t = 0.0
+    for k in range(3):
+        t += 1.0
+        g.halfstep_uz(t)
+        o.halfstep_uz(t)
+        rep = diff_report(g, o, HYDRO)
+        assert all(v["bitwise"] for v in rep.values()), ("uz", k, rep)
+        t += 1.0
+        g.halfstep_vz(t)
+        o.halfstep_vz(t)
+        rep = diff_report(g, o, HYDRO)
+        assert all(v["bitwise"] for v in rep.values()), ("vz", k, rep)
+
+
+def test_sediment_and_bed_change():
+    """Hydraulics stay bitwise with sediment on (nothing hydraulic reads FS);
+    FS, Kx, Ky and dH agree to the stated tolerance (device log/pow)."""
+    sc = S.open_box(44, 30, seed=11)
+    _print_case("sediment", name=sc.name)
+    g, o = make_pair(sc, sediment=True, bed_change=True)
+    g.step(10)
+    o.step(10)
+    rep = diff_report(g, o, HYDRO + ["VTH"])
+    assert all(v["bitwise"] for v in rep.values()), rep
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for f, rtol in (("FS", 1e-10), ("KX", 1e-12), ("KY", 1e-12), ("DH", 1e-9), ("TFS", 1e-10)):
+        a, b = g.get(f), o.get(f)
+        scale = np.abs(b).max() + 1e-300
+        assert np.max(np.abs(a - b)) <= rtol * scale, (f, np.max(np.abs(a - b)), scale)
+    assert np.abs(o.get("DH")).max() > 0 and np.abs(o.get("FS")).max() > 0
+
+
+def test_lake_at_rest_gpu():
+    sc = S.closed_basin(40, 30)
+    g, o = make_pair(sc)
+    g.step(10)
+    for f in ("Z", "U", "V"):
+        assert not np.any(g.get(f)), f
+
+
+def test_mass_balance_gpu():
+    sc = S.closed_basin(48, 36, z0_amp=0.2)
+    g, o = make_pair(sc)
+    v0, _ = g.mass_balance()
+    g.step(20)
+    o.step(20)
+    v1, q = g.mass_balance()
+    vo, qo = o.mass_balance()
+    assert q == 0.0
+    assert abs(v1 - v0) < 1e-9 * v0
+    assert abs(v1 - vo) < 1e-12 * vo
