@@ -192,8 +192,8 @@ int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
     if (ctx->cfg.sediment && t >= ops.sediment_start) {
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
-        LAUNCH(ctx, "k_fs_sweep<j>",
-               k_fs_sweep<1><<<dim3((s.M - 3 + 63) / 64, n), 64, 0, st>>>(S, 3, s.M - 1, t, ops.bed_change_start));
+        rc = run_line_sweep(ctx, S, s, n, 1, false, 3, s.M - 1, t, ops.bed_change_start);
+        if (rc) return rc;
         LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
     }
     return HD_OK;
@@ -220,8 +220,8 @@ int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     if (rc) return rc;
     if (ctx->cfg.sediment && t >= ops.sediment_start) {
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
-        LAUNCH(ctx, "k_fs_sweep<i>",
-               k_fs_sweep<0><<<dim3((s.N - 3 + 63) / 64, n), 64, 0, st>>>(S, 3, s.N - 1, t, ops.bed_change_start));
+        rc = run_line_sweep(ctx, S, s, n, 0, false, 3, s.N - 1, t, ops.bed_change_start);
+        if (rc) return rc;
         LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
         if (ctx->cfg.bed_change && t >= ops.bed_change_start) {
             LAUNCH(ctx, "k_qb", k_qb<<<g.grid, g.block, 0, st>>>(S));
@@ -232,17 +232,7 @@ int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
 }
 
 int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end) {
-    hipStream_t st = ctx->stream;
-    const int lines = end - start + 1;
-    if (lines <= 0) return HD_OK;
-    if (ctx->cfg.form == HD_FORM_FP64_PLAIN) {
-        if (dir == 1)
-            LAUNCH(ctx, "k_sweep_plain<uz>", k_sweep_plain<1><<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, start, end));
-        else
-            LAUNCH(ctx, "k_sweep_plain<vz>", k_sweep_plain<0><<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, start, end));
-        return HD_OK;
-    }
-    return run_sweep_fast(ctx, S, s, n, dir, start, end);
+    return run_line_sweep(ctx, S, s, n, dir, true, start, end, 0.0, 0.0);
 }
 
 }  // namespace hd

@@ -3,6 +3,7 @@
 // loader.cpp:224-375, with every buffer zero-initialised).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "hd_internal.h"
@@ -83,6 +84,19 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
     A((void **)&s.Ap, scratch * sizeof(double));
     A((void **)&s.Bp, scratch * sizeof(double));
     A((void **)&s.acc, (8 + (size_t)s.H) * sizeof(double));
+    // line-sweep buffers: ROW_MAJOR [rho][line] and LINE_MAJOR [line][rho]
+    SweepBufs &B = ctx->sweep;
+    const int rowsU = 2 * (s.N - 1), rowsV = 2 * (s.M - 1);
+    B.ld_rm[1] = round_up(s.M + 3, 64);
+    B.ld_rm[0] = round_up(s.N + 3, 64);
+    B.ld_lm[1] = round_up(rowsU, 64);
+    B.ld_lm[0] = round_up(rowsV, 64);
+    const size_t sw = std::max(std::max((size_t)rowsU * B.ld_rm[1], (size_t)rowsV * B.ld_rm[0]),
+                               std::max((size_t)(s.M + 3) * B.ld_lm[1], (size_t)(s.N + 3) * B.ld_lm[0]));
+    for (double **q : {&B.A, &B.B, &B.C, &B.D, &B.X, &B.Ap, &B.Bp}) A((void **)q, sw * sizeof(double));
+    const size_t nsys = std::max((size_t)s.capV * (s.M + 3), (size_t)s.capH * (s.N + 3));
+    A((void **)&B.sys0, nsys * sizeof(int));
+    A((void **)&B.sysn, nsys * sizeof(int));
     A((void **)&ctx->ds, sizeof(DevState));
     if (rc) return rc;
     s.h = h; s.hsnham = hsn; s.visc = vis; s.Fw = fw; s.Tsxw = tx; s.Tsyw = ty;
@@ -108,10 +122,5 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
     return HD_OK;
 }
 
-// Fast forms are added separately; until then they report unsupported.
-int run_sweep_fast(hd_ctx *ctx, const DevState *, const DevState &, int, int, int, int) {
-    ctx->err = "fast sweep forms are not built";
-    return HD_E_UNSUPPORTED;
-}
 
 }  // namespace hd

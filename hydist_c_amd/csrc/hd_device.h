@@ -346,4 +346,143 @@ HD_DEV void wall_flags(const DevState &s, int dir, int line, int first, int last
     }
 }
 
+// ---------------------------------------------------------------------------
+// Segment system structure (_calculate_matrix_coeff,
+// UVZSolver_multithread.cpp:345-470): layout of the first row, last row
+// index sn and the assembly's end type.
+// ---------------------------------------------------------------------------
+enum { END_WALL = 0, END_Q = 1, END_LONG = 2 };
+
+struct SegInfo {
+    int f, l, sn;
+    bool ran1, ran2, isB;
+    int mat_end;  // END_* used by the assembly
+};
+
+template <int DIR>
+HD_DEV SegInfo seg_info(const DevState &s, int line, int f, int l, bool dkQ1, bool dkQ2, bool q11) {
+    SegInfo g;
+    g.f = f;
+    g.l = l;
+    const int L = DIR == 1 ? s.N : s.M;
+    wall_flags(s, DIR, line, f, l, g.ran1, g.ran2);
+    int sn = 2 * (l - f);
+    if (g.ran1) {
+        g.isB = true;
+        if (!g.ran2) {
+            if (dkQ2 && l == L) {
+                g.mat_end = END_Q;
+            } else {
+                sn--;
+                g.mat_end = END_LONG;
+            }
+        } else {
+            g.mat_end = END_WALL;
+        }
+    } else {
+        g.isB = dkQ1 && f == 2;
+        const bool endQ = q11 ? dkQ1 : dkQ2;
+        if (!g.ran2) {
+            if (endQ && l == L) {
+                if (!g.isB) sn--;
+                g.mat_end = END_Q;
+            } else {
+                sn--;
+                if (!g.isB) sn -= 1;
+                g.mat_end = END_LONG;
+            }
+        } else {
+            if (!g.isB) sn--;
+            g.mat_end = END_WALL;
+        }
+    }
+    g.sn = sn;
+    return g;
+}
+
+template <int DIR>
+HD_DEV Pre preindex(const DevState &s, int line, int p) {
+    return DIR == 1 ? uz_preindex(s, p, line) : vz_preindex(s, line, p);
+}
+
+
+// FS row of cell (i, j) of a column segment [first, last]
+// (_FSj_calculate__mactrix_coeff, sediment_transport.cpp:204-313).
+HD_DEV void fs_row_j(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    const double gamav = 0.98 - 0.198 * 0.0 + 0.032 * 0.0 * 0.0;  // formed before Zf (:247)
+    FSCell fc = fs_cell(s, pos, t, s_start);
+    A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos - W] * Kx[pos - W] / (H[pos] * (dX * dX));
+    Cc = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos] * Kx[pos] / (H[pos] * (dX * dX));
+    B = 2.0 / dT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (H[pos] * (dX * dX));
+    double p = 0, q = 0;
+    if ((H[pos - 1] > H_TINH) && (H[pos + 2] > H_TINH)) p = (FS[pos + 1] - FS[pos]);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 1] > H_TINH)) q = (FS[pos] - FS[pos - 1]);
+    p = (1 / (H[pos] * (dY * dY))) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 2] > H_TINH)) {
+        q = (FS[pos + 1] - FS[pos - 1]) / (dY * 2);
+        q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
+    } else
+        q = 0;
+    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
+    if (i == first + 1) {
+        if ((ran1) || (t_u[first * W + j] == 0))
+            B += A;
+        else
+            D -= A * FS[first * W + j];
+    }
+    if (i == last - 1) {
+        if ((ran2) && (t_u[last * W + j] == 0))
+            B += Cc;
+        else
+            D -= Cc * FS[last * W + j];
+    }
+}
+
+// FS row of cell (i, j) of a row segment
+// (_FSi_calculate__mactrix_coeff, sediment_transport.cpp:51-164).
+HD_DEV void fs_row_i(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    FSCell fc = fs_cell(s, pos, t, s_start);
+    const double gamav = 0.98 - 0.198 * fc.Zf + 0.032 * fc.Zf * fc.Zf;
+    A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos - 1] * Ky[pos - 1] / (H[pos] * (dY * dY));
+    Cc = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos] * Ky[pos] / (H[pos] * (dY * dY));
+    B = 2.0 / dT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (H[pos] * (dY * dY));
+    double p = 0, q = 0;
+    if ((H[pos + 2 * W] > H_TINH) && (H[pos - W] > H_TINH)) p = (FS[pos + W] - FS[pos]);
+    if (H[pos - 2 * W] > H_TINH && H[pos + W] > H_TINH) q = (FS[pos] - FS[pos - W]);
+    p = (1 / (H[pos] * (dX * dX))) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+    if (H[pos + 2 * W] > H_TINH && H[pos - 2 * W] > H_TINH) {
+        q = (FS[pos + W] - FS[pos - W]) / (dX * 2);
+        q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
+    } else
+        q = 0;
+    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
+    if (j == first + 1) {
+        if ((ran1) || (t_v[i * W + first] == 0))
+            B = B + A;
+        else
+            D = D - A * FS[i * W + first];
+    }
+    if (j == last - 1) {
+        if ((ran2) || (t_v[i * W + last] == 0))
+            B = B + Cc;
+        else
+            D = D - Cc * FS[i * W + last];
+    }
+}
+
+
 }  // namespace hd

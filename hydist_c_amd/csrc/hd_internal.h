@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "hd_kernels.h"
+#include "hd_sweep.h"
 
 struct hd_ctx {
     int device = 0;
@@ -19,6 +20,7 @@ struct hd_ctx {
     hd_config cfg{};
     hd::DevState hs{};             // descriptor (host copy)
     hd::DevState *ds = nullptr;    // descriptor (device copy)
+    hd::SweepBufs sweep{};         // line-sweep buffers
     std::vector<void *> allocs;
     size_t bytes = 0;
     bool inited = false;
@@ -67,6 +69,5 @@ int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
 int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
 int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end);
-int run_sweep_fast(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end);
 
 }  // namespace hd

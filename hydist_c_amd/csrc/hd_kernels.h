@@ -11,7 +11,6 @@ __global__ void k_segment_cols(const DevState *S);
 __global__ void k_htuongdoi(const DevState *S);
 __global__ void k_preprocess(const DevState *S);
 __global__ void k_boundary(const DevState *S, double t);
-template <int DIR> __global__ void k_sweep_plain(const DevState *S, int start, int end);
 template <bool ISU> __global__ void k_smooth(const DevState *S);
 __global__ void k_solve_v(const DevState *S);
 __global__ void k_solve_u(const DevState *S);
@@ -23,7 +22,6 @@ __global__ void k_dry(const DevState *S);
 __global__ void k_post_b(const DevState *S);
 __global__ void k_interp_fs(const DevState *S);
 __global__ void k_vth_k(const DevState *S);
-template <int DIR> __global__ void k_fs_sweep(const DevState *S, int start, int end, double t, double s_start);
 __global__ void k_update_fs(const DevState *S);
 __global__ void k_qb(const DevState *S);
 __global__ void k_bedload(const DevState *S);

@@ -246,261 +246,6 @@ __global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Plain FP64 implicit sweep: the interleaved (z, vel) system of every wet
-// segment, assembled and solved by Thomas in the reference's row order
-// (UVZSolver_multithread.cpp:19-43, :110-139, :280-603).
-//
-// One lane per line; the lanes of a wavefront walk the line positions in
-// lockstep, so DIR == 1 (UZ: line = column j, walk along i) reads each row
-// of the state coalesced.  Forward pass: coefficients + elimination, Ap/Bp
-// for two rows per cell in scratch laid out [2*p + slot][line].  Backward
-// pass: back-substitution and extraction into t_z and t_u (UZ) / t_v (VZ).
-// ---------------------------------------------------------------------------
-enum { END_WALL = 0, END_Q = 1, END_LONG = 2 };
-
-struct SegInfo {
-    int f, l, sn;
-    bool ran1, ran2, isB;
-    int mat_end;  // END_* used by the assembly
-};
-
-template <int DIR>
-HD_DEV SegInfo seg_info(const DevState &s, int line, int f, int l, bool dkQ1, bool dkQ2, bool q11) {
-    SegInfo g;
-    g.f = f;
-    g.l = l;
-    const int L = DIR == 1 ? s.N : s.M;
-    wall_flags(s, DIR, line, f, l, g.ran1, g.ran2);
-    int sn = 2 * (l - f);
-    if (g.ran1) {
-        g.isB = true;
-        if (!g.ran2) {
-            if (dkQ2 && l == L) {
-                g.mat_end = END_Q;
-            } else {
-                sn--;
-                g.mat_end = END_LONG;
-            }
-        } else {
-            g.mat_end = END_WALL;
-        }
-    } else {
-        g.isB = dkQ1 && f == 2;
-        const bool endQ = q11 ? dkQ1 : dkQ2;
-        if (!g.ran2) {
-            if (endQ && l == L) {
-                if (!g.isB) sn--;
-                g.mat_end = END_Q;
-            } else {
-                sn--;
-                if (!g.isB) sn -= 1;
-                g.mat_end = END_LONG;
-            }
-        } else {
-            if (!g.isB) sn--;
-            g.mat_end = END_WALL;
-        }
-    }
-    g.sn = sn;
-    return g;
-}
-
-template <int DIR>
-HD_DEV Pre preindex(const DevState &s, int line, int p) {
-    return DIR == 1 ? uz_preindex(s, p, line) : vz_preindex(s, line, p);
-}
-
-template <int DIR>
-__global__ void __launch_bounds__(64) k_sweep_plain(const DevState *__restrict__ S, int start, int end) {
-    const DevState &s = S[blockIdx.y];
-    const int line = start + blockIdx.x * 64 + threadIdx.x;
-    const bool active = line <= end;
-    const int L = DIR == 1 ? s.N : s.M;
-    const int W = s.P;
-    const int ld = DIR == 1 ? s.P : s.ldT;
-    const int *seg_n = DIR == 1 ? s.segV_n : s.segH_n;
-    const int cap = DIR == 1 ? s.capV : s.capH;
-    const int *segf = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)(active ? line : 0) * cap;
-    const int *segl = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)(active ? line : 0) * cap;
-    const int nseg = active ? seg_n[line] : 0;
-    const double f4 = DIR == 1 ? 2 * s.c.g * s.c.dTchia2dX : 2 * s.c.g * s.c.dTchia2dY;
-    const bool dkQ1 = DIR == 1 ? s.bienQ[2] : s.bienQ[1];
-    const bool dkQ2 = DIR == 1 ? s.bienQ[3] : s.bienQ[0];
-    const bool q11 = (s.quirks & HD_QUIRK_Q11_LITERAL) != 0;
-    const double ub_start = active ? (DIR == 1 ? s.ubt[line] : s.vbd[line]) : 0.0;
-    const double ub_end = active ? (DIR == 1 ? s.ubp[line] : s.vbt[line]) : 0.0;
-    double *__restrict__ Ap = s.Ap;
-    double *__restrict__ Bp = s.Bp;
-    double *__restrict__ tz = s.t_z;
-    double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
-    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
-    auto sidx = [&](int p, int slot) { return (size_t)(2 * p + slot) * ld + line; };
-
-    // ---------------- forward ----------------
-    {
-        int k = 0;
-        SegInfo g;
-        g.f = nseg > 0 ? segf[0] : L + 1;
-        g.l = nseg > 0 ? segl[0] : L;
-        g.sn = -1;
-        g.isB = false;
-        g.ran1 = g.ran2 = false;
-        g.mat_end = END_WALL;
-        double TZ_l = 0, TZ_r = 0, ApP = 0, BpP = 0;
-        Pre cur = {}, nxt = {};
-        for (int p = 2; p <= L; p++) {
-            if (k < nseg && p > g.l) {
-                k++;
-                if (k < nseg) {
-                    g.f = segf[k];
-                    g.l = segl[k];
-                }
-            }
-            if (!(k < nseg && p >= g.f && p <= g.l)) continue;
-            const int f = g.f, l = g.l;
-            if (p == f) {
-                g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
-                TZ_l = tz[cell(f)];
-                TZ_r = tz[cell(l)];
-                cur = preindex<DIR>(s, line, f);
-            }
-            const int sn = g.sn;
-            if (p + 1 <= l) nxt = preindex<DIR>(s, line, p + 1);
-            const int r0 = 2 * (p - f);
-            if (r0 > sn) {
-                cur = nxt;
-                continue;
-            }
-            const bool need_mom = p <= l - 1;
-            Mom m = {0, 0, 0, 0};
-            if (need_mom) m = calc_abcd(p, f, l, f4, g.ran1, g.ran2, cur, nxt);
-            double A[2], B[2], C[2], D[2];
-            if (g.isB) {
-                A[0] = cur.a2; B[0] = 1; C[0] = cur.c2; D[0] = cur.d2;
-                if (p == f && !g.ran1) D[0] = cur.d2 - cur.a2 * ub_start;
-                A[1] = m.a1; B[1] = m.b1; C[1] = m.c1; D[1] = m.d1;
-            } else {
-                A[0] = m.a1; B[0] = m.b1; C[0] = m.c1; D[0] = m.d1;
-                if (p == f) D[0] = m.d1 - m.a1 * TZ_l;
-                A[1] = nxt.a2; B[1] = 1; C[1] = nxt.c2; D[1] = nxt.d2;
-            }
-            for (int slot = 0; slot < 2; slot++) {
-                const int r = r0 + slot;
-                if (r > sn) break;
-                double a = A[slot], b = B[slot], cc = C[slot], d = D[slot];
-                if (r == sn) {
-                    // end row (the last write wins over the per-cell row)
-                    if (g.mat_end == END_WALL) {
-                        const Pre &e = g.isB ? cur : nxt;  // continuity of cell l
-                        a = e.a2; b = 1; d = e.d2;
-                    } else if (g.mat_end == END_Q) {
-                        const Pre &e = g.isB ? cur : nxt;
-                        a = e.a2; b = 1; d = e.d2 - e.c2 * ub_end;
-                    } else {
-                        a = m.a1; b = m.b1; d = m.d1 - m.c1 * TZ_r;
-                    }
-                }
-                double ap, bp;
-                if (r == 0) {
-                    if (sn == 0) {
-                        ap = 0;
-                        bp = d / b;
-                    } else {
-                        ap = -cc / b;
-                        bp = d / b;
-                    }
-                } else if (r < sn) {
-                    const double ep = a * ApP + b;
-                    ap = -cc / ep;
-                    bp = (d - (a * BpP)) / ep;
-                } else {
-                    ap = 0;
-                    bp = (d - (a * BpP)) / (b + (a * ApP));
-                }
-                Ap[sidx(p, slot)] = ap;
-                Bp[sidx(p, slot)] = bp;
-                ApP = ap;
-                BpP = bp;
-            }
-            cur = nxt;
-        }
-    }
-
-    // ---------------- backward ----------------
-    {
-        int k = nseg - 1;
-        SegInfo g;
-        g.f = nseg > 0 ? segf[k] : L + 1;
-        g.l = nseg > 0 ? segl[k] : L;
-        g.sn = -1;
-        double xnext = 0, x_sn = 0, tvel_lm1 = 0;
-        for (int p = L; p >= 2; p--) {
-            if (k >= 0 && p < g.f) {
-                k--;
-                if (k >= 0) {
-                    g.f = segf[k];
-                    g.l = segl[k];
-                }
-            }
-            if (!(k >= 0 && p >= g.f && p <= g.l)) continue;
-            const int f = g.f, l = g.l;
-            if (p == l) g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
-            const int sn = g.sn;
-            const int r0 = 2 * (p - f);
-            double x[2] = {0, 0};
-            for (int slot = 1; slot >= 0; slot--) {
-                const int r = r0 + slot;
-                if (r > sn) continue;
-                const double bp = Bp[sidx(p, slot)];
-                double xv;
-                if (r == sn) {
-                    xv = bp;
-                    x_sn = bp;
-                } else {
-                    xv = bp + (Ap[sidx(p, slot)] * xnext);
-                }
-                x[slot] = xv;
-                xnext = xv;
-            }
-            // interior cells (p < l)
-            if (p < l) {
-                if (g.isB) {
-                    tz[cell(p)] = x[0];
-                    tv[cell(p)] = x[1];
-                    if (p == l - 1) tvel_lm1 = x[1];
-                    if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : ub_start;
-                } else {
-                    tv[cell(p)] = x[0];
-                    if (p == l - 1) tvel_lm1 = x[0];
-                    if (p + 1 < l) tz[cell(p + 1)] = x[1];
-                    if (p == f) {
-                        const Pre e = preindex<DIR>(s, line, f);
-                        tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * x[0]) / e.a2;
-                    }
-                }
-            }
-            // end cell, once t_vel[l-1] and x[sn] are known
-            if (p == (l > f ? l - 1 : l)) {
-                const bool ext_Q = dkQ2 && l == L;
-                if (g.ran2) {
-                    tv[cell(l)] = 0;
-                    if (sn >= 0) tz[cell(l)] = x_sn;
-                } else if (ext_Q) {
-                    tv[cell(l)] = ub_end;
-                    if (sn >= 0) tz[cell(l)] = x_sn;
-                } else {
-                    const Pre e = preindex<DIR>(s, line, l);
-                    const double prev = l > f ? tvel_lm1 : tv[cell(l - 1)];
-                    tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
-                }
-            }
-        }
-    }
-}
-
-template __global__ void k_sweep_plain<0>(const DevState *, int, int);
-template __global__ void k_sweep_plain<1>(const DevState *, int, int);
 
 // ---------------------------------------------------------------------------
 // Smoothing (Normalize, support_funcs.cpp:761-808): tmp = t_vel everywhere,
@@ -810,190 +555,6 @@ __global__ void k_vth_k(const DevState *__restrict__ S) {
     }
 }
 
-// FS row of cell (i, j) of a column segment [first, last]
-// (_FSj_calculate__mactrix_coeff, sediment_transport.cpp:204-313).
-HD_DEV void fs_row_j(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
-                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
-    const int W = s.P;
-    const hd_coeffs &c = s.c;
-    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
-    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
-    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
-    const int pos = i * W + j;
-    const double gamav = 0.98 - 0.198 * 0.0 + 0.032 * 0.0 * 0.0;  // formed before Zf (:247)
-    FSCell fc = fs_cell(s, pos, t, s_start);
-    A = -gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos - W] * Kx[pos - W] / (H[pos] * (dX * dX));
-    Cc = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) / (dX * 2) - Htdu[pos] * Kx[pos] / (H[pos] * (dX * dX));
-    B = 2.0 / dT + (Htdu[pos] * Kx[pos] + Htdu[pos - W] * Kx[pos - W]) / (H[pos] * (dX * dX));
-    double p = 0, q = 0;
-    if ((H[pos - 1] > H_TINH) && (H[pos + 2] > H_TINH)) p = (FS[pos + 1] - FS[pos]);
-    if ((H[pos - 2] > H_TINH) && (H[pos + 1] > H_TINH)) q = (FS[pos] - FS[pos - 1]);
-    p = (1 / (H[pos] * (dY * dY))) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
-    if ((H[pos - 2] > H_TINH) && (H[pos + 2] > H_TINH)) {
-        q = (FS[pos + 1] - FS[pos - 1]) / (dY * 2);
-        q = (t_v[pos] + t_v[pos - 1]) * 0.5 * q * gamav;
-    } else
-        q = 0;
-    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
-    if (i == first + 1) {
-        if ((ran1) || (t_u[first * W + j] == 0))
-            B += A;
-        else
-            D -= A * FS[first * W + j];
-    }
-    if (i == last - 1) {
-        if ((ran2) && (t_u[last * W + j] == 0))
-            B += Cc;
-        else
-            D -= Cc * FS[last * W + j];
-    }
-}
-
-// FS row of cell (i, j) of a row segment
-// (_FSi_calculate__mactrix_coeff, sediment_transport.cpp:51-164).
-HD_DEV void fs_row_i(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
-                     double t, double s_start, double &A, double &B, double &Cc, double &D) {
-    const int W = s.P;
-    const hd_coeffs &c = s.c;
-    const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
-    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
-    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
-    const int pos = i * W + j;
-    FSCell fc = fs_cell(s, pos, t, s_start);
-    const double gamav = 0.98 - 0.198 * fc.Zf + 0.032 * fc.Zf * fc.Zf;
-    A = -gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos - 1] * Ky[pos - 1] / (H[pos] * (dY * dY));
-    Cc = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) / (dY * 2) - Htdv[pos] * Ky[pos] / (H[pos] * (dY * dY));
-    B = 2.0 / dT + (Htdv[pos] * Ky[pos] + Htdv[pos - 1] * Ky[pos - 1]) / (H[pos] * (dY * dY));
-    double p = 0, q = 0;
-    if ((H[pos + 2 * W] > H_TINH) && (H[pos - W] > H_TINH)) p = (FS[pos + W] - FS[pos]);
-    if (H[pos - 2 * W] > H_TINH && H[pos + W] > H_TINH) q = (FS[pos] - FS[pos - W]);
-    p = (1 / (H[pos] * (dX * dX))) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
-    if (H[pos + 2 * W] > H_TINH && H[pos - 2 * W] > H_TINH) {
-        q = (FS[pos + W] - FS[pos - W]) / (dX * 2);
-        q = (t_u[pos] + t_u[pos - W]) * 0.5 * q * gamav;
-    } else
-        q = 0;
-    D = FS[pos] / (dT * 0.5) - q + p + (fc.S / H[pos]);
-    if (j == first + 1) {
-        if ((ran1) || (t_v[i * W + first] == 0))
-            B = B + A;
-        else
-            D = D - A * FS[i * W + first];
-    }
-    if (j == last - 1) {
-        if ((ran2) || (t_v[i * W + last] == 0))
-            B = B + Cc;
-        else
-            D = D - Cc * FS[i * W + last];
-    }
-}
-
-// FS sweep: Thomas per segment of at least three cells, one lane per line
-// (DIR 1: columns j in [start, end], along i; DIR 0: rows, along j), with
-// the extraction of _FSj/_FSi_extract_solution (:316-349, :167-201) into tFS.
-template <int DIR>
-__global__ void __launch_bounds__(64) k_fs_sweep(const DevState *__restrict__ S, int start, int end, double t,
-                                                  double s_start) {
-    const DevState &s = S[blockIdx.y];
-    const int line = start + blockIdx.x * 64 + threadIdx.x;
-    const bool active = line <= end;
-    const int L = DIR == 1 ? s.N : s.M;
-    const int W = s.P;
-    const int ld = DIR == 1 ? s.P : s.ldT;
-    const int cap = DIR == 1 ? s.capV : s.capH;
-    const int *segf = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)(active ? line : 0) * cap;
-    const int *segl = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)(active ? line : 0) * cap;
-    const int nseg = active ? (DIR == 1 ? s.segV_n : s.segH_n)[line] : 0;
-    double *__restrict__ Ap = s.Ap;
-    double *__restrict__ Bp = s.Bp;
-    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
-    auto sidx = [&](int p) { return (size_t)p * ld + line; };
-    // forward
-    {
-        int k = 0, f = nseg > 0 ? segf[0] : L + 1, l = nseg > 0 ? segl[0] : L;
-        bool ran1 = false, ran2 = false;
-        double ApP = 0, BpP = 0;
-        for (int p = 2; p <= L; p++) {
-            if (k < nseg && p > l) {
-                k++;
-                if (k < nseg) {
-                    f = segf[k];
-                    l = segl[k];
-                }
-            }
-            if (!(k < nseg && p >= f + 1 && p <= l - 1)) continue;
-            if (p == f + 1) wall_flags(s, DIR, line, f, l, ran1, ran2);
-            const int sn = l - f - 2, r = p - f - 1;
-            double a, b, cc, d;
-            if (DIR == 1)
-                fs_row_j(s, p, line, f, l, ran1, ran2, t, s_start, a, b, cc, d);
-            else
-                fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, a, b, cc, d);
-            double ap, bp;
-            if (r == 0) {
-                if (sn == 0) {
-                    ap = 0;
-                    bp = d / b;
-                } else {
-                    ap = -cc / b;
-                    bp = d / b;
-                }
-            } else if (r < sn) {
-                const double ep = a * ApP + b;
-                ap = -cc / ep;
-                bp = (d - (a * BpP)) / ep;
-            } else {
-                ap = 0;
-                bp = (d - (a * BpP)) / (b + (a * ApP));
-            }
-            Ap[sidx(p)] = ap;
-            Bp[sidx(p)] = bp;
-            ApP = ap;
-            BpP = bp;
-        }
-    }
-    // backward + extraction
-    {
-        int k = nseg - 1, f = nseg > 0 ? segf[k] : L + 1, l = nseg > 0 ? segl[k] : L;
-        bool ran1 = false, ran2 = false;
-        double xnext = 0;
-        double *tFS = s.tFS;
-        const double *FS = s.FS;
-        const double *tvel = DIR == 1 ? s.t_u : s.t_v;
-        const double NDnen = s.c.NDnen;
-        for (int p = L; p >= 2; p--) {
-            if (k >= 0 && p < f) {
-                k--;
-                if (k >= 0) {
-                    f = segf[k];
-                    l = segl[k];
-                }
-            }
-            if (!(k >= 0 && p >= f + 1 && p <= l - 1)) continue;
-            if (p == l - 1) wall_flags(s, DIR, line, f, l, ran1, ran2);
-            const int sn = l - f - 2, r = p - f - 1;
-            const double bp = Bp[sidx(p)];
-            const double xv = (r == sn) ? bp : bp + (Ap[sidx(p)] * xnext);
-            xnext = xv;
-            const double out = xv < 0 ? NDnen : xv;
-            tFS[cell(p)] = out;
-            if (p == l - 1) {
-                if ((ran2) || (tvel[cell(l)] == 0))
-                    tFS[cell(l)] = out;
-                else
-                    tFS[cell(l)] = FS[cell(l)];
-            }
-            if (p == f + 1) {
-                if ((ran1) || (tvel[cell(f)] == 0))
-                    tFS[cell(f)] = out;
-                else
-                    tFS[cell(f)] = FS[cell(f)];
-            }
-        }
-    }
-}
-template __global__ void k_fs_sweep<0>(const DevState *, int, int, double, double);
-template __global__ void k_fs_sweep<1>(const DevState *, int, int, double, double);
 
 // Update_FS (sediment_transport.cpp:654-663).
 __global__ void k_update_fs(const DevState *__restrict__ S) {

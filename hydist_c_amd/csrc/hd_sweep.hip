@@ -1,0 +1,815 @@
+// hd_sweep.hip - implicit line sweeps of the ADI timestep on gfx950.
+//
+// A sweep is split into three kernels:
+//   assemble: one thread per cell forms the rows of every wet segment's
+//             tridiagonal system (UVZSolver_multithread.cpp:110-470 for the
+//             hydraulics, sediment_transport.cpp:51-313 for FS);
+//   solve:    either Thomas in the reference's row order, one lane per line
+//             (bitwise FP64, UVZSolver_multithread.cpp:19-43), or cyclic
+//             reduction, one workgroup per line (fast forms);
+//   extract:  one thread per cell writes the solution back into t_z and
+//             t_u / t_v (UVZSolver_multithread.cpp:472-603) or tFS
+//             (sediment_transport.cpp:167-201, :316-349).
+//
+// Row space of a line: position p in [2, L] (L = N along columns, M along
+// rows) owns row rho = 2(p-2) for z_p and rho = 2(p-2)+1 for the velocity
+// at face p (hydraulics), or rho = p-2 (FS).  A segment's system is the
+// contiguous row range [rho0, rho0 + sn]; every other row is an identity
+// row, so a line is one block-diagonal tridiagonal system.  The first row
+// of a system has A = 0 and the last C = 0 (the reference never reads
+// those entries).
+//
+// Layouts: ROW_MAJOR keeps row rho of all lines adjacent ([rho][line]) so
+// the lanes of a wavefront (one line each) read coalesced; LINE_MAJOR keeps
+// each line contiguous ([line][rho]) for the workgroup-per-line solver.
+#include "hd_internal.h"
+
+namespace hd {
+
+// ---------------------------------------------------------------------------
+// Segment description shared by assembly and extraction
+// ---------------------------------------------------------------------------
+struct SegCtx {
+    SegInfo g;
+    double TZ_l, TZ_r, ub_start, ub_end;
+    int rho0;  // first row of the system (-1 when sn < 0)
+};
+
+template <int DIR>
+HD_DEV bool locate(const DevState &s, int line, int p, int &k, int &f, int &l) {
+    const int n = DIR == 1 ? s.segV_n[line] : s.segH_n[line];
+    const int cap = DIR == 1 ? s.capV : s.capH;
+    const int *fs = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)line * cap;
+    const int *ls = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)line * cap;
+    k = find_segment(fs, ls, n, p);
+    if (k < 0) return false;
+    f = fs[k];
+    l = ls[k];
+    return true;
+}
+
+template <int DIR>
+HD_DEV SegCtx seg_ctx(const DevState &s, int line, int f, int l) {
+    SegCtx c;
+    const bool dkQ1 = DIR == 1 ? s.bienQ[2] : s.bienQ[1];
+    const bool dkQ2 = DIR == 1 ? s.bienQ[3] : s.bienQ[0];
+    const bool q11 = (s.quirks & HD_QUIRK_Q11_LITERAL) != 0;
+    c.g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
+    const int W = s.P;
+    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
+    c.TZ_l = s.t_z[cell(f)];
+    c.TZ_r = s.t_z[cell(l)];
+    c.ub_start = DIR == 1 ? s.ubt[line] : s.vbd[line];
+    c.ub_end = DIR == 1 ? s.ubp[line] : s.vbt[line];
+    c.rho0 = c.g.sn < 0 ? -1 : 2 * (f - 2) + (c.g.isB ? 0 : 1);
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// Hydraulic assembly: rows 2(p-2) (z_p) and 2(p-2)+1 (velocity at p).
+// ---------------------------------------------------------------------------
+struct Row {
+    double a, b, c, d;
+};
+
+HD_DEV Row identity_row() { return Row{0.0, 1.0, 0.0, 0.0}; }
+
+// The two rows owned by position p of its segment (see the layout rules in
+// the reference's bienrandau / bienlongdau and end overrides).
+template <int DIR>
+HD_DEV void hyd_rows(const DevState &s, int line, int p, const SegCtx &sc, double f4, Row &rz, Row &ru) {
+    const SegInfo &g = sc.g;
+    const int f = g.f, l = g.l, sn = g.sn;
+    rz = identity_row();
+    ru = identity_row();
+    if (sn < 0) return;
+    // local row indices (relative to the system's first row)
+    const int base = g.isB ? 2 * f : 2 * f + 1;  // global 2p maps to local 2p - base
+    const int rz_loc = 2 * p - base, ru_loc = 2 * p + 1 - base;
+    const bool hz = rz_loc >= 0 && rz_loc <= sn;
+    const bool hu = ru_loc >= 0 && ru_loc <= sn;
+    if (!hz && !hu) return;
+    const Pre cur = preindex<DIR>(s, line, p);
+    Pre nxt = cur;
+    if (p + 1 <= l) nxt = preindex<DIR>(s, line, p + 1);
+    if (hz) {
+        // continuity of cell p
+        Row r{cur.a2, 1.0, cur.c2, cur.d2};
+        if (rz_loc == 0 && g.isB && !g.ran1) r.d = cur.d2 - cur.a2 * sc.ub_start;
+        if (rz_loc == sn) {
+            if (g.mat_end == END_Q) r.d = cur.d2 - cur.c2 * sc.ub_end;
+            else r.d = cur.d2;  // END_WALL (END_LONG never ends on a continuity row)
+        }
+        if (rz_loc == 0) r.a = 0.0;
+        if (rz_loc == sn) r.c = 0.0;
+        rz = r;
+    }
+    if (hu) {
+        const Mom m = calc_abcd(p, f, l, f4, g.ran1, g.ran2, cur, nxt);
+        Row r{m.a1, m.b1, m.c1, m.d1};
+        if (ru_loc == 0 && !g.isB) r.d = m.d1 - m.a1 * sc.TZ_l;
+        if (ru_loc == sn) r.d = m.d1 - m.c1 * sc.TZ_r;  // END_LONG
+        if (ru_loc == 0) r.a = 0.0;
+        if (ru_loc == sn) r.c = 0.0;
+        ru = r;
+    }
+}
+
+// Row-major index of row rho of line `line`; line-major index.
+HD_DEV size_t rm_idx(int rho, int line, int ld) { return (size_t)rho * ld + line; }
+HD_DEV size_t lm_idx(int rho, int line, int ld) { return (size_t)line * ld + rho; }
+
+// Threads: x along j (coalesced state reads), y along i.  For outputs whose
+// layout runs against the thread order, a 64 x 16 tile is transposed
+// through LDS.
+template <int DIR, int LAYOUT>
+__global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict__ S, SweepBufs B, int start,
+                                                      int end) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 8 + threadIdx.y;
+    const int line = DIR == 1 ? j : i;
+    const int p = DIR == 1 ? i : j;
+    const int L = DIR == 1 ? s.N : s.M;
+    const bool active = line >= start && line <= end && p >= 2 && p <= L;
+    const double f4 = DIR == 1 ? 2 * s.c.g * s.c.dTchia2dX : 2 * s.c.g * s.c.dTchia2dY;
+    Row rz = identity_row(), ru = identity_row();
+    if (active) {
+        int k, f, l;
+        if (locate<DIR>(s, line, p, k, f, l)) {
+            const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
+            hyd_rows<DIR>(s, line, p, sc, f4, rz, ru);
+            if (p == f) {
+                const int cap = DIR == 1 ? s.capV : s.capH;
+                B.sys0[(size_t)line * cap + k] = sc.rho0;
+                B.sysn[(size_t)line * cap + k] = sc.g.sn;
+            }
+        }
+    }
+    // Output orientation: the thread grid runs along j.  UZ lines are
+    // columns (line = j): ROW_MAJOR [rho][line] is coalesced directly.
+    // VZ lines are rows (line = i, rho from j): LINE_MAJOR [line][rho] is
+    // coalesced directly.  The other two combinations go through LDS.
+    constexpr bool direct = (DIR == 1 && LAYOUT == ROW_MAJOR) || (DIR == 0 && LAYOUT == LINE_MAJOR);
+    if (direct) {
+        if (!active) return;
+        const int rz_i = 2 * (p - 2), ru_i = rz_i + 1;
+        const int ld = B.ld;
+        size_t iz, iu;
+        if (LAYOUT == ROW_MAJOR) {
+            iz = rm_idx(rz_i, line, ld);
+            iu = rm_idx(ru_i, line, ld);
+        } else {
+            iz = lm_idx(rz_i, line, ld);
+            iu = lm_idx(ru_i, line, ld);
+        }
+        B.A[iz] = rz.a; B.B[iz] = rz.b; B.C[iz] = rz.c; B.D[iz] = rz.d;
+        B.A[iu] = ru.a; B.B[iu] = ru.b; B.C[iu] = ru.c; B.D[iu] = ru.d;
+    } else {
+        // tile: 8 (i) x 64 (j) cells, 2 rows each -> transpose so that the
+        // runs along i become contiguous
+        __shared__ double tA[8][129], tB[8][129], tC[8][129], tD[8][129];
+        const int ty = threadIdx.y, tx = threadIdx.x;
+        tA[ty][2 * tx] = rz.a; tB[ty][2 * tx] = rz.b; tC[ty][2 * tx] = rz.c; tD[ty][2 * tx] = rz.d;
+        tA[ty][2 * tx + 1] = ru.a; tB[ty][2 * tx + 1] = ru.b; tC[ty][2 * tx + 1] = ru.c; tD[ty][2 * tx + 1] = ru.d;
+        __syncthreads();
+        // write out: for DIR==1 & LINE_MAJOR: line = j (64 lines), rho from i (8 cells -> 16 rows)
+        //            for DIR==0 & ROW_MAJOR:  line = i (8 lines), rho from j (64 cells -> 128 rows)
+        const int t = ty * 64 + tx;  // 0..511
+        const int i0 = blockIdx.y * 8, j0 = blockIdx.x * 64;
+        const int ld = B.ld;
+        if (DIR == 1) {
+            // 64 lines x 16 rows = 1024 values per array, 2 per thread
+            for (int e = t; e < 64 * 16; e += 512) {
+                const int ln = e / 16, r = e % 16;  // r runs fastest -> contiguous in [line][rho]
+                const int jj = j0 + ln, ci = i0 + r / 2, slot = r & 1;
+                if (jj < start || jj > end || ci < 2 || ci > L) continue;
+                const size_t o = lm_idx(2 * (ci - 2) + slot, jj, ld);
+                const int col = 2 * ln + slot;
+                B.A[o] = tA[r / 2][col]; B.B[o] = tB[r / 2][col]; B.C[o] = tC[r / 2][col]; B.D[o] = tD[r / 2][col];
+            }
+        } else {
+            // 8 lines x 128 rows; line index fastest for coalescing in [rho][line]
+            for (int e = t; e < 8 * 128; e += 512) {
+                const int ln = e % 8, r = e / 8;
+                const int ii = i0 + ln, cj = j0 + r / 2, slot = r & 1;
+                if (ii < start || ii > end || cj < 2 || cj > L) continue;
+                const size_t o = rm_idx(2 * (cj - 2) + slot, ii, ld);
+                B.A[o] = tA[ln][r]; B.B[o] = tB[ln][r]; B.C[o] = tC[ln][r]; B.D[o] = tD[ln][r];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FS assembly: row p-2 for cells in [f+1, l-1] of segments of >= 3 cells.
+// ---------------------------------------------------------------------------
+template <int DIR, int LAYOUT>
+__global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict__ S, SweepBufs B, int start,
+                                                     int end, double t, double s_start) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int i = blockIdx.y * 8 + threadIdx.y;
+    const int line = DIR == 1 ? j : i;
+    const int p = DIR == 1 ? i : j;
+    const int L = DIR == 1 ? s.N : s.M;
+    const bool active = line >= start && line <= end && p >= 2 && p <= L;
+    Row r = identity_row();
+    if (active) {
+        int k, f, l;
+        if (locate<DIR>(s, line, p, k, f, l) && l - f >= 2) {
+            bool ran1, ran2;
+            wall_flags(s, DIR, line, f, l, ran1, ran2);
+            if (p >= f + 1 && p <= l - 1) {
+                if (DIR == 1)
+                    fs_row_j(s, p, line, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                else
+                    fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                if (p == f + 1) r.a = 0.0;
+                if (p == l - 1) r.c = 0.0;
+            }
+            if (p == f) {
+                const int cap = DIR == 1 ? s.capV : s.capH;
+                B.sys0[(size_t)line * cap + k] = f + 1 - 2;
+                B.sysn[(size_t)line * cap + k] = l - f - 2;
+            }
+        } else if (k >= 0 && p == f) {
+            const int cap = DIR == 1 ? s.capV : s.capH;
+            B.sys0[(size_t)line * cap + k] = -1;
+            B.sysn[(size_t)line * cap + k] = -1;
+        }
+    }
+    constexpr bool direct = (DIR == 1 && LAYOUT == ROW_MAJOR) || (DIR == 0 && LAYOUT == LINE_MAJOR);
+    if (direct) {
+        if (!active) return;
+        const size_t o = LAYOUT == ROW_MAJOR ? rm_idx(p - 2, line, B.ld) : lm_idx(p - 2, line, B.ld);
+        B.A[o] = r.a; B.B[o] = r.b; B.C[o] = r.c; B.D[o] = r.d;
+    } else {
+        __shared__ double tA[8][65], tB[8][65], tC[8][65], tD[8][65];
+        const int ty = threadIdx.y, tx = threadIdx.x;
+        tA[ty][tx] = r.a; tB[ty][tx] = r.b; tC[ty][tx] = r.c; tD[ty][tx] = r.d;
+        __syncthreads();
+        const int t = ty * 64 + tx;
+        const int i0 = blockIdx.y * 8, j0 = blockIdx.x * 64;
+        const int ln = DIR == 1 ? t / 8 : t % 8;
+        const int rr = DIR == 1 ? t % 8 : t / 8;
+        if (DIR == 1) {  // line = j (64), rho from i (8), LINE_MAJOR
+            const int jj = j0 + ln, ci = i0 + rr;
+            if (jj < start || jj > end || ci < 2 || ci > L) return;
+            const size_t o = lm_idx(ci - 2, jj, B.ld);
+            B.A[o] = tA[rr][ln]; B.B[o] = tB[rr][ln]; B.C[o] = tC[rr][ln]; B.D[o] = tD[rr][ln];
+        } else {  // line = i (8), rho from j (64), ROW_MAJOR
+            const int ii = i0 + ln, cj = j0 + rr;
+            if (ii < start || ii > end || cj < 2 || cj > L) return;
+            const size_t o = rm_idx(cj - 2, ii, B.ld);
+            B.A[o] = tA[ln][rr]; B.B[o] = tB[ln][rr]; B.C[o] = tC[ln][rr]; B.D[o] = tD[ln][rr];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Thomas, one lane per line, rows of all lines in lockstep ([rho][line]).
+// Rows are read in blocks of TB with the next block's loads issued before
+// the current block's serial recurrence, so memory latency overlaps it.
+// Arithmetic: tridiag (UVZSolver_multithread.cpp:19-43), unchanged.
+// ---------------------------------------------------------------------------
+constexpr int TB = 8;
+
+__global__ void __launch_bounds__(64) k_thomas(const DevState *__restrict__ S, SweepBufs B, int dir, int start,
+                                               int end, int nrows) {
+    const DevState &s = S[blockIdx.y];
+    const int line = start + blockIdx.x * 64 + threadIdx.x;
+    const bool active = line <= end;
+    const int ld = B.ld;
+    const int cap = dir == 1 ? s.capV : s.capH;
+    const int nseg = active ? (dir == 1 ? s.segV_n : s.segH_n)[line] : 0;
+    const int *sys0 = B.sys0 + (size_t)(active ? line : 0) * cap;
+    const int *sysn = B.sysn + (size_t)(active ? line : 0) * cap;
+    const double *__restrict__ A = B.A + (active ? line : 0);
+    const double *__restrict__ Bv = B.B + (active ? line : 0);
+    const double *__restrict__ C = B.C + (active ? line : 0);
+    const double *__restrict__ D = B.D + (active ? line : 0);
+    double *__restrict__ Ap = B.Ap + (active ? line : 0);
+    double *__restrict__ Bp = B.Bp + (active ? line : 0);
+    double *__restrict__ X = B.X + (active ? line : 0);
+
+    // forward
+    int k = 0, r0 = 0, sn = -1;
+    auto next_sys = [&]() {
+        while (k < nseg && sysn[k] < 0) k++;
+        if (k < nseg) {
+            r0 = sys0[k];
+            sn = sysn[k];
+        } else {
+            r0 = nrows;
+            sn = -1;
+        }
+    };
+    next_sys();
+    double ApP = 0, BpP = 0;
+    double ca[TB], cb[TB], cc[TB], cd[TB];
+    double na[TB], nb[TB], nc[TB], nd[TB];
+    for (int q = 0; q < TB; q++) {
+        const int rho = q;
+        const bool ok = active && rho < nrows;
+        na[q] = ok ? A[(size_t)rho * ld] : 0;
+        nb[q] = ok ? Bv[(size_t)rho * ld] : 1;
+        nc[q] = ok ? C[(size_t)rho * ld] : 0;
+        nd[q] = ok ? D[(size_t)rho * ld] : 0;
+    }
+    for (int blk = 0; blk < nrows; blk += TB) {
+        for (int q = 0; q < TB; q++) {
+            ca[q] = na[q]; cb[q] = nb[q]; cc[q] = nc[q]; cd[q] = nd[q];
+        }
+        for (int q = 0; q < TB; q++) {
+            const int rho = blk + TB + q;
+            const bool ok = active && rho < nrows;
+            na[q] = ok ? A[(size_t)rho * ld] : 0;
+            nb[q] = ok ? Bv[(size_t)rho * ld] : 1;
+            nc[q] = ok ? C[(size_t)rho * ld] : 0;
+            nd[q] = ok ? D[(size_t)rho * ld] : 0;
+        }
+        for (int q = 0; q < TB; q++) {
+            const int rho = blk + q;
+            if (rho >= nrows) break;
+            if (k < nseg && rho >= r0 && rho <= r0 + sn) {
+                const int r = rho - r0;
+                const double a = ca[q], b = cb[q], c = cc[q], d = cd[q];
+                double ap, bp;
+                if (r == 0) {
+                    if (sn == 0) {
+                        ap = 0;
+                        bp = d / b;
+                    } else {
+                        ap = -c / b;
+                        bp = d / b;
+                    }
+                } else if (r < sn) {
+                    const double ep = a * ApP + b;
+                    ap = -c / ep;
+                    bp = (d - (a * BpP)) / ep;
+                } else {
+                    ap = 0;
+                    bp = (d - (a * BpP)) / (b + (a * ApP));
+                }
+                Ap[(size_t)rho * ld] = ap;
+                Bp[(size_t)rho * ld] = bp;
+                ApP = ap;
+                BpP = bp;
+                if (r == sn) {
+                    k++;
+                    next_sys();
+                }
+            }
+        }
+    }
+    // backward
+    k = nseg - 1;
+    auto prev_sys = [&]() {
+        while (k >= 0 && sysn[k] < 0) k--;
+        if (k >= 0) {
+            r0 = sys0[k];
+            sn = sysn[k];
+        } else {
+            r0 = -1;
+            sn = -1;
+        }
+    };
+    prev_sys();
+    double xnext = 0;
+    const int top = ((nrows + TB - 1) / TB) * TB;
+    double pa[TB], pb[TB], qa[TB], qb[TB];
+    for (int q = 0; q < TB; q++) {
+        const int rho = top - 1 - q;
+        const bool ok = active && rho < nrows && rho >= 0;
+        qa[q] = ok ? Ap[(size_t)rho * ld] : 0;
+        qb[q] = ok ? Bp[(size_t)rho * ld] : 0;
+    }
+    for (int blk = top - 1; blk >= 0; blk -= TB) {
+        for (int q = 0; q < TB; q++) {
+            pa[q] = qa[q];
+            pb[q] = qb[q];
+        }
+        for (int q = 0; q < TB; q++) {
+            const int rho = blk - TB - q;
+            const bool ok = active && rho >= 0 && rho < nrows;
+            qa[q] = ok ? Ap[(size_t)rho * ld] : 0;
+            qb[q] = ok ? Bp[(size_t)rho * ld] : 0;
+        }
+        for (int q = 0; q < TB; q++) {
+            const int rho = blk - q;
+            if (rho < 0) break;
+            if (k >= 0 && rho >= r0 && rho <= r0 + sn) {
+                const int r = rho - r0;
+                const double xv = (r == sn) ? pb[q] : pb[q] + (pa[q] * xnext);
+                X[(size_t)rho * ld] = xv;
+                xnext = xv;
+                if (r == 0) {
+                    k--;
+                    prev_sys();
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cyclic reduction, one workgroup per line ([line][rho]).
+//
+// Hydraulics (INTERLEAVED): the velocity rows (odd rho) are first
+// eliminated into the z rows, giving the z-only system of the line
+// (strictly diagonally dominant), which is solved by cyclic reduction in
+// precision T; with REFINE, the FP64 residual of that system is solved once
+// more (reusing the stored elimination multipliers) and added.  The
+// velocities are then recovered in FP64 from their own rows.
+// FS (!INTERLEAVED): the line's system is solved directly.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CRShared {
+    T *a, *b, *c, *d, *k1, *k2;
+};
+
+// Forward reduction + back substitution of n rows held in LDS.  When
+// `store` is set, the multipliers of every update are kept in k1/k2 so a
+// second right-hand side can be reduced with cr_replay.
+template <typename T>
+HD_DEV void cr_solve(CRShared<T> sh, int n, bool store) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int K = 0;
+    while ((2 << K) <= n) K++;  // 2^K <= n < 2^(K+1)
+    int off = 0;
+    for (int lev = 1; lev <= K; lev++) {
+        const int h = 1 << (lev - 1), st = 1 << lev;
+        const int cnt = n >> lev;
+        for (int m = tid; m < cnt; m += nt) {
+            const int i = (m + 1) * st - 1;
+            const int il = i - h, ir = i + h;
+            const T k1 = sh.a[i] / sh.b[il];
+            T k2 = 0;
+            T nb = sh.b[i] - sh.c[il] * k1;
+            T nd = sh.d[i] - sh.d[il] * k1;
+            T na = -sh.a[il] * k1;
+            T nc = 0;
+            if (ir < n) {
+                k2 = sh.c[i] / sh.b[ir];
+                nb = nb - sh.a[ir] * k2;
+                nd = nd - sh.d[ir] * k2;
+                nc = -sh.c[ir] * k2;
+            }
+            sh.a[i] = na;
+            sh.b[i] = nb;
+            sh.c[i] = nc;
+            sh.d[i] = nd;
+            if (store) {
+                sh.k1[off + m] = k1;
+                sh.k2[off + m] = k2;
+            }
+        }
+        off += cnt;
+        __syncthreads();
+    }
+    // top row
+    if (tid == 0) {
+        const int i = (1 << K) - 1;
+        sh.d[i] = sh.d[i] / sh.b[i];
+    }
+    __syncthreads();
+    for (int lev = K; lev >= 1; lev--) {
+        const int h = 1 << (lev - 1), st = 1 << lev;
+        // rows (i+1) % h == 0 and (i+1) % st != 0
+        const int cnt = (n + h) / st;  // i = (2m+1)h - 1 < n
+        for (int m = tid; m < cnt; m += nt) {
+            const int i = (2 * m + 1) * h - 1;
+            if (i >= n) continue;
+            const int il = i - h, ir = i + h;
+            T v = sh.d[i];
+            if (il >= 0) v = v - sh.a[i] * sh.d[il];
+            if (ir < n) v = v - sh.c[i] * sh.d[ir];
+            sh.d[i] = v / sh.b[i];
+        }
+        __syncthreads();
+    }
+}
+
+// Reduce and solve a new right-hand side in sh.d with the multipliers and
+// final rows left by cr_solve(store = true).
+template <typename T>
+HD_DEV void cr_replay(CRShared<T> sh, int n) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int K = 0;
+    while ((2 << K) <= n) K++;
+    int off = 0;
+    for (int lev = 1; lev <= K; lev++) {
+        const int h = 1 << (lev - 1), st = 1 << lev;
+        const int cnt = n >> lev;
+        for (int m = tid; m < cnt; m += nt) {
+            const int i = (m + 1) * st - 1;
+            const int il = i - h, ir = i + h;
+            T nd = sh.d[i] - sh.d[il] * sh.k1[off + m];
+            if (ir < n) nd = nd - sh.d[ir] * sh.k2[off + m];
+            sh.d[i] = nd;
+        }
+        off += cnt;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int i = (1 << K) - 1;
+        sh.d[i] = sh.d[i] / sh.b[i];
+    }
+    __syncthreads();
+    for (int lev = K; lev >= 1; lev--) {
+        const int h = 1 << (lev - 1), st = 1 << lev;
+        const int cnt = (n + h) / st;
+        for (int m = tid; m < cnt; m += nt) {
+            const int i = (2 * m + 1) * h - 1;
+            if (i >= n) continue;
+            const int il = i - h, ir = i + h;
+            T v = sh.d[i];
+            if (il >= 0) v = v - sh.a[i] * sh.d[il];
+            if (ir < n) v = v - sh.c[i] * sh.d[ir];
+            sh.d[i] = v / sh.b[i];
+        }
+        __syncthreads();
+    }
+}
+
+// z-row q of an interleaved line (rows 2q-1, 2q, 2q+1) reduced in FP64.
+HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__ Bv,
+                        const double *__restrict__ C, const double *__restrict__ D, int q, int nrows) {
+    const int r = 2 * q;
+    Row o{0.0, Bv[r], 0.0, D[r]};
+    const double ar = A[r], cr = C[r];
+    if (r - 1 >= 0 && ar != 0.0) {
+        const double al = ar / Bv[r - 1];
+        o.a = -al * A[r - 1];
+        o.b = o.b - al * C[r - 1];
+        o.d = o.d - al * D[r - 1];
+    }
+    if (r + 1 < nrows && cr != 0.0) {
+        const double ga = cr / Bv[r + 1];
+        o.c = -ga * C[r + 1];
+        o.b = o.b - ga * A[r + 1];
+        o.d = o.d - ga * D[r + 1];
+    }
+    return o;
+}
+
+template <typename T, bool REFINE, bool INTERLEAVED>
+__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs B, int start, int end,
+                                           int nrows) {
+    extern __shared__ unsigned char smem[];
+    const int line = start + blockIdx.x;
+    if (line > end) return;
+    const int ld = B.ld;
+    const double *__restrict__ A = B.A + (size_t)line * ld;
+    const double *__restrict__ Bv = B.B + (size_t)line * ld;
+    const double *__restrict__ C = B.C + (size_t)line * ld;
+    const double *__restrict__ D = B.D + (size_t)line * ld;
+    double *__restrict__ X = B.X + (size_t)line * ld;
+    // FP64 reduced system kept for the residual (global scratch Ap/Bp: a,b | c,d)
+    double *__restrict__ Ra = B.Ap + (size_t)line * ld;
+    double *__restrict__ Rb = Ra + ld / 2;
+    double *__restrict__ Rc = B.Bp + (size_t)line * ld;
+    double *__restrict__ Rd = Rc + ld / 2;
+    const int n = INTERLEAVED ? nrows / 2 : nrows;
+    T *base = reinterpret_cast<T *>(smem);
+    CRShared<T> sh{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n};
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < n; q += nt) {
+        Row r;
+        if (INTERLEAVED)
+            r = reduce_z_row(A, Bv, C, D, q, nrows);
+        else
+            r = Row{A[q], Bv[q], C[q], D[q]};
+        sh.a[q] = (T)r.a;
+        sh.b[q] = (T)r.b;
+        sh.c[q] = (T)r.c;
+        sh.d[q] = (T)r.d;
+        if (REFINE) {
+            Ra[q] = r.a;
+            Rb[q] = r.b;
+            Rc[q] = r.c;
+            Rd[q] = r.d;
+        }
+    }
+    __syncthreads();
+    cr_solve<T>(sh, n, REFINE);
+    if (REFINE) {
+        // z0 -> X (FP64), residual of the FP64 system -> d
+        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+        __syncthreads();
+        for (int q = tid; q < n; q += nt) {
+            const int stp = INTERLEAVED ? 2 : 1;
+            const double xq = X[stp * q];
+            double rr = Rd[q] - Rb[q] * xq;
+            if (q > 0) rr = rr - Ra[q] * X[stp * (q - 1)];
+            if (q + 1 < n) rr = rr - Rc[q] * X[stp * (q + 1)];
+            sh.d[q] = (T)rr;
+        }
+        __syncthreads();
+        cr_replay<T>(sh, n);
+        for (int q = tid; q < n; q += nt) {
+            const int stp = INTERLEAVED ? 2 : 1;
+            X[stp * q] = X[stp * q] + (double)sh.d[q];
+        }
+    } else {
+        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+    }
+    if (INTERLEAVED) {
+        __syncthreads();
+        // velocity rows from their own FP64 rows
+        for (int q = tid; q < n; q += nt) {
+            const int r = 2 * q + 1;
+            if (r >= nrows) continue;
+            double v = D[r];
+            const double a = A[r], c = C[r];
+            if (a != 0.0) v = v - a * X[r - 1];
+            if (c != 0.0 && r + 1 < nrows) v = v - c * X[r + 1];
+            X[r] = v / Bv[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Extraction (hydraulics).  Define: cells before `last` are written before
+// the end cell, whose continuity formula uses the new velocity at last-1.
+// ---------------------------------------------------------------------------
+template <int DIR, int LAYOUT>
+__global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs B, int start, int end) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int line = DIR == 1 ? j : i;
+    const int p = DIR == 1 ? i : j;
+    const int L = DIR == 1 ? s.N : s.M;
+    if (line < start || line > end || p < 2 || p > L) return;
+    int k, f, l;
+    if (!locate<DIR>(s, line, p, k, f, l)) return;
+    const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
+    const SegInfo &g = sc.g;
+    const int W = s.P, ld = B.ld;
+    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
+    auto xat = [&](int rho) {
+        return LAYOUT == ROW_MAJOR ? B.X[rm_idx(rho, line, ld)] : B.X[lm_idx(rho, line, ld)];
+    };
+    double *__restrict__ tz = s.t_z;
+    double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
+    const int sn = g.sn;
+    const int rz = 2 * (p - 2), ru = rz + 1;
+    if (p < l) {
+        if (g.isB) {
+            tz[cell(p)] = xat(rz);
+            tv[cell(p)] = xat(ru);
+            if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : sc.ub_start;
+        } else {
+            tv[cell(p)] = xat(ru);
+            if (p > f) tz[cell(p)] = xat(rz);
+            if (p == f) {
+                const Pre e = preindex<DIR>(s, line, f);
+                tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * xat(ru)) / e.a2;
+            }
+        }
+    } else {  // p == l
+        const bool ext_Q = (DIR == 1 ? s.bienQ[3] : s.bienQ[0]) && l == L;
+        if (g.ran2) {
+            tv[cell(l)] = 0;
+            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
+        } else if (ext_Q) {
+            tv[cell(l)] = sc.ub_end;
+            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
+        } else {
+            const Pre e = preindex<DIR>(s, line, l);
+            const double prev = l > f ? xat(2 * (l - 1 - 2) + 1) : tv[cell(l - 1)];
+            tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
+        }
+    }
+}
+
+// FS extraction (sediment_transport.cpp:167-201, :316-349): negative values
+// become NDnen; the segment's end cells take the neighbouring value or keep
+// FS as the reference's end tests decide.
+template <int DIR, int LAYOUT>
+__global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs B, int start, int end) {
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int line = DIR == 1 ? j : i;
+    const int p = DIR == 1 ? i : j;
+    const int L = DIR == 1 ? s.N : s.M;
+    if (line < start || line > end || p < 2 || p > L) return;
+    int k, f, l;
+    if (!locate<DIR>(s, line, p, k, f, l) || l - f < 2) return;
+    if (p < f + 1 || p > l - 1) return;
+    bool ran1, ran2;
+    wall_flags(s, DIR, line, f, l, ran1, ran2);
+    const int W = s.P, ld = B.ld;
+    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
+    const double xv = LAYOUT == ROW_MAJOR ? B.X[rm_idx(p - 2, line, ld)] : B.X[lm_idx(p - 2, line, ld)];
+    const double out = xv < 0 ? s.c.NDnen : xv;
+    const double *tvel = DIR == 1 ? s.t_u : s.t_v;
+    s.tFS[cell(p)] = out;
+    if (p == f + 1) {
+        if ((ran1) || (tvel[cell(f)] == 0))
+            s.tFS[cell(f)] = out;
+        else
+            s.tFS[cell(f)] = s.FS[cell(f)];
+    }
+    if (p == l - 1) {
+        if ((ran2) || (tvel[cell(l)] == 0))
+            s.tFS[cell(l)] = out;
+        else
+            s.tFS[cell(l)] = s.FS[cell(l)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host sequencing of one sweep
+// ---------------------------------------------------------------------------
+static dim3 tile_grid(const DevState &s, int n) { return dim3((s.M + 3 + 63) / 64, (s.N + 3 + 7) / 8, n); }
+
+int sweep_rows(const DevState &s, int dir, bool hyd) {
+    const int L = dir == 1 ? s.N : s.M;
+    return hyd ? 2 * (L - 1) : (L - 1);
+}
+
+// Lets a kernel use more than the default 64 KiB of dynamic LDS (gfx950
+// has 160 KiB per CU).
+template <typename K>
+static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
+    if (bytes <= 65536) return HD_OK;
+    hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) {
+        ctx->err = std::string("hipFuncSetAttribute(LDS): ") + hipGetErrorString(e);
+        return HD_E_HIP;
+    }
+    return HD_OK;
+}
+
+template <int DIR, int LAYOUT>
+static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
+                      double t, double s_start) {
+    hipStream_t st = ctx->stream;
+    SweepBufs B = ctx->sweep;
+    const int nrows = sweep_rows(s, DIR, hyd);
+    B.ld = LAYOUT == ROW_MAJOR ? B.ld_rm[DIR] : B.ld_lm[DIR];
+    const dim3 tg = tile_grid(s, n);
+    if (hyd)
+        LAUNCH(ctx, DIR ? "k_hyd_assemble<uz>" : "k_hyd_assemble<vz>",
+               k_hyd_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end));
+    else
+        LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
+               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start));
+    const int lines = end - start + 1;
+    const int form = ctx->cfg.form;
+    if (LAYOUT == ROW_MAJOR) {
+        LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
+               k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
+    } else {
+        const int nz = hyd ? nrows / 2 : nrows;
+        const int threads = nz > 1024 ? 512 : 256;
+        if (form == HD_FORM_FP32_REFINE) {
+            const size_t sm = 6 * (size_t)nz * sizeof(float);
+            if (n != 1) return HD_E_UNSUPPORTED;
+            int rc = allow_lds(ctx, hyd ? k_cr<float, true, true> : k_cr<float, true, false>, sm);
+            if (rc) return rc;
+            if (hyd)
+                LAUNCH(ctx, DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>",
+                       (k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+            else
+                LAUNCH(ctx, DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>",
+                       (k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+        } else {
+            const size_t sm = 4 * (size_t)nz * sizeof(double);
+            if (n != 1) return HD_E_UNSUPPORTED;
+            int rc = allow_lds(ctx, hyd ? k_cr<double, false, true> : k_cr<double, false, false>, sm);
+            if (rc) return rc;
+            if (hyd)
+                LAUNCH(ctx, DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>",
+                       (k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+            else
+                LAUNCH(ctx, DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>",
+                       (k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+        }
+    }
+    const Launch2D g = grid2d(s, n);
+    if (hyd)
+        LAUNCH(ctx, DIR ? "k_hyd_extract<uz>" : "k_hyd_extract<vz>",
+               (k_hyd_extract<DIR, LAYOUT><<<g.grid, g.block, 0, st>>>(S, B, start, end)));
+    else
+        LAUNCH(ctx, DIR ? "k_fs_extract<j>" : "k_fs_extract<i>",
+               (k_fs_extract<DIR, LAYOUT><<<g.grid, g.block, 0, st>>>(S, B, start, end)));
+    return HD_OK;
+}
+
+int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
+                   int end, double t, double s_start) {
+    if (end < start) return HD_OK;
+    const bool plain = ctx->cfg.form == HD_FORM_FP64_PLAIN;
+    if (dir == 1)
+        return plain ? sweep_impl<1, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
+                     : sweep_impl<1, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
+    return plain ? sweep_impl<0, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
+                 : sweep_impl<0, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
+}
+
+}  // namespace hd

@@ -255,12 +255,16 @@ def open_box(M, N, seed=3, n_islands=3, depth=(3.0, 6.0), Q_up=-150.0, Q_left=-8
     shape = h.shape
     total_time = hours + 3
     t = np.arange(total_time, dtype=np.float64)
-    bcs = [
-        _series(Q_up, total_time) * (1 + 0.1 * np.sin(t)),
-        z_down * np.cos(0.7 * t),
-        _series(Q_left, total_time),
-        z_right + 0.02 * np.sin(0.5 * t),
-    ]
+    # per side: a discharge series when the side is Q-type (2), else a
+    # water-level series
+    q_mag = [Q_up, -Q_up, Q_left, -Q_left]
+    z_mag = [0.03, z_down, 0.02, z_right + 0.01]
+    bcs = []
+    for side in range(4):
+        if types[side] == 2:
+            bcs.append(np.full(total_time, q_mag[side]) * (1 + 0.1 * np.sin(t + side)))
+        else:
+            bcs.append(z_mag[side] * np.cos(0.7 * t + side))
     bcs = [np.ascontiguousarray(b, dtype=np.float64) for b in bcs]
     cc = [rng.uniform(0.05, 0.2, size=total_time) for _ in range(4)] if fs else None
     return Scenario(
@@ -397,3 +401,33 @@ def _runs(line, end):
                 out.append(end - start + 1)
                 start = end
     return out
+
+
+def tidal_flats(M, N, seed=6, shelf=(0.03, 0.16), channel_depth=3.0, drop=2.0, rise_after=True,
+                name=None):
+    """Channel along j flanked by a shallow shelf (depth `shelf`); the down
+    side (j = 2) water level falls by `drop` m in the first hour (and rises
+    back in the second), so shelf cells dry out and later re-wet
+    (Reset_states_vertical, support_funcs.cpp:142-188)."""
+    rng = np.random.default_rng(seed)
+    yc = N / 2.0
+
+    def in_channel(y, x):
+        return (y > 2) & (y < N - 3)
+
+    def dep(y, x):
+        r = np.abs(y - yc) / (0.5 * N)
+        shelf_d = shelf[0] + (shelf[1] - shelf[0]) * rng.uniform(size=np.shape(y))
+        return np.where(r < 0.35, channel_depth * (1 - r / 0.35) + shelf[1] * (r / 0.35), shelf_d)
+
+    h = build_bathymetry(M, N, in_channel, dep)
+    shape = h.shape
+    total_time = 5
+    z_down = np.array([0.0, -drop, 0.0 if rise_after else -drop, 0.0, 0.0])
+    zeros = _series(0.0, total_time)
+    return Scenario(
+        name=name or f"flats_{M}x{N}", M=M, N=N, h=h, hsnham=rng.uniform(0.025, 0.035, size=shape),
+        visc=np.full(shape, 5.0), Fw=np.full(shape, 0.003), total_time=total_time,
+        boundary_type=(0, 0, 0, 0), bc=[zeros, np.ascontiguousarray(z_down), zeros, zeros], CC=None,
+        meta={"seed": seed},
+    )

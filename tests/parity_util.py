@@ -12,8 +12,15 @@ HYDRO = ["Z", "U", "V", "TZ", "TU", "TV", "HMOI", "HTDU", "HTDV"]
 
 
 def bits_equal(a, b):
-    """Bitwise equality of float64 arrays (NaNs compare by payload)."""
-    return np.array_equal(np.ascontiguousarray(a).view(np.int64), np.ascontiguousarray(b).view(np.int64))
+    """Bitwise equality of float64 arrays.  NaNs must sit at the same
+    places; their sign/payload is not compared (x86 and the GPU produce
+    default NaNs of opposite sign)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return np.array_equal(np.where(na, 0.0, a).view(np.int64), np.where(nb, 0.0, b).view(np.int64))
 
 
 def ulp_diff(a, b):

@@ -16,8 +16,8 @@ from parity_util import HYDRO, bits_equal, diff_report, make_pair, ulp_diff
 pytestmark = pytest.mark.gpu
 
 
-def _print_case(name, **kw):
-    print(f"[case] {name} {kw}", flush=True)
+def _print_case(case, **kw):
+    print(f"[case] {case} {kw}", flush=True)
 
 
 @pytest.mark.parametrize("maker", [lambda: S.open_box(40, 28), lambda: S.config1(),
@@ -50,6 +50,7 @@ def test_init_matches_oracle(maker):
     (lambda: S.config1(), 30),
     (lambda: S.meander_islands(96, 64, n_islands=6), 20),
     (lambda: S.closed_basin(48, 36, z0_amp=0.3), 25),
+    (lambda: S.tidal_flats(64, 40), 200),
 ])
 def test_plain_fp64_bitwise(maker, steps):
     sc = maker()
@@ -119,6 +120,19 @@ def test_lake_at_rest_gpu():
     g.step(10)
     for f in ("Z", "U", "V"):
         assert not np.any(g.get(f)), f
+
+
+def test_plain_fp64_one_hour_config1_bitwise():
+    """Config 1 for one simulated hour (1800 steps): still bit for bit."""
+    sc = S.config1()
+    _print_case("plain_1h", name=sc.name)
+    g, o = make_pair(sc)
+    for _ in range(18):
+        g.step(100)
+        o.step(100)
+    rep = diff_report(g, o, HYDRO)
+    assert all(v["bitwise"] for v in rep.values()), rep
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
 
 
 def test_mass_balance_gpu():
