@@ -47,6 +47,23 @@ def stage_bytes(name, sim, sc):
     if name.startswith("k_sweep_plain"):
         d = 1 if "uz" in name else 0
         return 40.0 * sweep_unknowns(sim, d)  # (a, b, c, d in + x out) * 8 B per unknown
+    # line sweeps over (L-1) positions of every line
+    lines_uz, lines_vz = sc.M - 1, sc.N - 1
+    pos_uz, pos_vz = sc.N - 1, sc.M - 1
+    if name.startswith(("k_cr", "k_thomas", "k_hyd_assemble", "k_hyd_extract", "k_fs_assemble", "k_fs_extract")):
+        uz = any(t in name for t in ("uz", "fs_j", "<j>"))
+        n = (lines_uz * pos_uz) if uz else (lines_vz * pos_vz)
+        hyd = "fs" not in name
+        if name.startswith(("k_cr", "k_thomas")):
+            # tridiagonal solve: 4 coefficients in + solution out per unknown
+            return (80.0 if hyd else 40.0) * n
+        if name.startswith("k_hyd_assemble"):
+            return (12 * 8 + 2 * 4 * 8) * float(n)
+        if name.startswith("k_fs_assemble"):
+            return (12 * 8 + 4 * 8) * float(n)
+        if name.startswith("k_hyd_extract"):
+            return 32.0 * n
+        return 24.0 * n
     per_cell = {
         "k_smooth": 8 + 4 + 8,        # t_vel + khouot in, tmp out
         "k_solve_v": 10 * 8 + 4 + 16,  # stencil fields in, t_v + t_u out
@@ -188,7 +205,10 @@ def main():
         "config": {"workload": "2048x1024 meander+islands, sediment + FP64 bed change, dt=2s",
                    "M": sc.M, "N": sc.N, "form": best, "timed_from_t": sc.sediment_start},
         "forms": {k: {"cell_steps_per_s": v["cell_steps_per_s"], "ms_per_step": v["ms_per_step"],
-                      "finite": v["finite"]} for k, v in results.items()},
+                      "finite": v["finite"],
+                      "profile_top": sorted(({"stage": a, "launches": b[0], "avg_ms": b[1] / b[0]}
+                                             for a, b in v["stats"].items()), key=lambda d: -d["avg_ms"])[:6]}
+                  for k, v in results.items()},
         "roofline": roofline,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,

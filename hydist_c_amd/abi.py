@@ -82,7 +82,7 @@ class Mesh(C.Structure):
 class Config(C.Structure):
     _fields_ = [
         ("form", C.c_int), ("sediment", C.c_int), ("bed_change", C.c_int),
-        ("quirks", C.c_int), ("device", C.c_int),
+        ("quirks", C.c_int), ("device", C.c_int), ("dense_layout", C.c_int),
     ]
 
 

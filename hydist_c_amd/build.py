@@ -36,6 +36,23 @@ def _compile(src):
     return obj
 
 
+def build_compat_driver(verbose=False):
+    """tests/compat/compat_driver: a program using the reference's host
+    surface (include/hydist_compat.h) linked against the library."""
+    src = os.path.join(ROOT, "tests", "compat", "compat_driver.cpp")
+    out_dir = os.path.join(ROOT, "tests", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, "compat_driver")
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), src, "-o", exe,
+           "-L" + PKG, "-lhydist_amd", "-Wl,-rpath," + PKG]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat driver build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -49,6 +66,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_compat_driver(verbose)
     return LIB
 
 

@@ -199,7 +199,7 @@ int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     return HD_OK;
 }
 
-int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
+int run_halfstep_vz_a(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
     hipStream_t st = ctx->stream;
     const hd_options &ops = ctx->ops;
     Launch2D g = grid2d(s, n);
@@ -215,8 +215,17 @@ int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     LAUNCH(ctx, "k_pre_reset", k_pre_reset<<<g.grid, g.block, 0, st>>>(S));
     LAUNCH(ctx, "k_rewet", k_rewet<<<dim3((s.M + 63) / 64, n), 64, 0, st>>>(S));
     LAUNCH(ctx, "k_dry", k_dry<<<g.grid, g.block, 0, st>>>(S));
+    return HD_OK;
+}
+
+// From update_uvz on (engine.cpp:247-288); save_result sits between the two
+// parts in the reference's loop (engine.cpp:243-244).
+int run_halfstep_vz_b(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
+    hipStream_t st = ctx->stream;
+    const hd_options &ops = ctx->ops;
+    Launch2D g = grid2d(s, n);
     LAUNCH(ctx, "k_post_b", k_post_b<<<g.grid, g.block, 0, st>>>(S));
-    rc = run_segmentation(ctx, S, s, n);
+    int rc = run_segmentation(ctx, S, s, n);
     if (rc) return rc;
     if (ctx->cfg.sediment && t >= ops.sediment_start) {
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
@@ -229,6 +238,12 @@ int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
         }
     }
     return HD_OK;
+}
+
+int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t) {
+    int rc = run_halfstep_vz_a(ctx, S, s, n, t);
+    if (rc) return rc;
+    return run_halfstep_vz_b(ctx, S, s, n, t);
 }
 
 int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end) {

@@ -32,7 +32,7 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
     DevState &s = ctx->hs;
     s.M = mesh->M;
     s.N = mesh->N;
-    s.P = round_up(s.M + 3, 32);
+    s.P = ctx->cfg.dense_layout ? s.M + 3 : round_up(s.M + 3, 32);
     s.H = s.N + 4;
     s.ldT = round_up(s.N + 3, 32);
     s.capH = (s.M + 3) / 2 + 2;

@@ -68,6 +68,8 @@ int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
 int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
+int run_halfstep_vz_a(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
+int run_halfstep_vz_b(hd_ctx *ctx, const DevState *S, const DevState &s, int n, double t);
 int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end);
 
 }  // namespace hd

@@ -82,7 +82,11 @@ __global__ void __launch_bounds__(256) k_segment_rows(const DevState *__restrict
 }
 
 // Columns (lines along i, stride P): one lane per column, lanes of a
-// wavefront read adjacent columns of the same row.
+// wavefront read adjacent columns of the same row.  Each lane loads a chunk
+// of SEGC rows with independent loads before scanning it, so the memory
+// latency is paid once per chunk.
+constexpr int SEGC = 32;
+
 __global__ void __launch_bounds__(64) k_segment_cols(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
@@ -92,14 +96,27 @@ __global__ void __launch_bounds__(64) k_segment_cols(const DevState *__restrict_
     int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
     int n = 0;
     bool prev_wet = false;
-    for (int p = 2; p <= end; p++) {
-        const bool wet = kh[p * W + j] == 0;
-        if (wet && !prev_wet) first[n] = p;
-        if (wet && (p == end || kh[(p + 1) * W + j] != 0)) {
-            last[n] = p;
-            n++;
+    int cur = kh[2 * W + j];
+    for (int c = 2; c <= end; c += SEGC) {
+        int nxt[SEGC];
+#pragma unroll
+        for (int q = 0; q < SEGC; q++) {
+            const int p = c + 1 + q;  // row p+1 of the scan, read ahead
+            nxt[q] = p <= end ? kh[p * W + j] : 2;
         }
-        prev_wet = wet;
+#pragma unroll
+        for (int q = 0; q < SEGC; q++) {
+            const int p = c + q;
+            if (p > end) break;
+            const bool wet = cur == 0;
+            if (wet && !prev_wet) first[n] = p;
+            if (wet && (p == end || nxt[q] != 0)) {
+                last[n] = p;
+                n++;
+            }
+            prev_wet = wet;
+            cur = nxt[q];
+        }
     }
     if (n > 0 && first[n - 1] >= end - 1) n--;
     s.segV_n[j] = n;
@@ -422,8 +439,13 @@ __global__ void k_pre_reset(const DevState *__restrict__ S) {
 }
 
 // Reset_states_vertical (support_funcs.cpp:142-188), re-wetting pass: one
-// lane per column j in [1, M], serial along i (lanes read adjacent columns).
-__global__ void k_rewet(const DevState *__restrict__ S) {
+// lane per column j in [1, M], serial along i (lanes read adjacent
+// columns).  t_z, z and khouot of a chunk of rows are read ahead into
+// registers; every update is written through to memory at once, so the
+// next chunk's loads and the rare H_moi / FS reads see it.
+constexpr int RWC = 32;
+
+__global__ void __launch_bounds__(64) k_rewet(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (j > s.M) return;
@@ -431,20 +453,40 @@ __global__ void k_rewet(const DevState *__restrict__ S) {
     double *t_z = s.t_z, *H_moi = s.H_moi, *FS = s.FS;
     const double *z = s.z, *htaiz = s.htaiz;
     int *kh = s.khouot;
-    for (int i = 2; i <= N; i++) {
-        const double tzi = t_z[i * W + j];
-        if (tzi > z[i * W + j]) {
-            if (kh[(i - 1) * W + j] == 1) {
-                t_z[(i - 1) * W + j] = tzi;
-                H_moi[(i - 1) * W + j] = htaiz[(i - 1) * W + j] + tzi;
-                kh[(i - 1) * W + j] = 0;
-                FS[(i - 1) * W + j] = FS[i * W + j];
-            }
-            if (kh[(i + 1) * W + j] == 1) {
-                t_z[(i + 1) * W + j] = tzi;
-                H_moi[(i + 1) * W + j] = htaiz[(i + 1) * W + j] + tzi;
-                kh[(i + 1) * W + j] = 0;
-                FS[(i + 1) * W + j] = FS[i * W + j];
+    for (int c = 2; c <= N; c += RWC) {
+        // window rows c-1 .. c+RWC
+        double tz[RWC + 2], zz[RWC + 2];
+        int kk[RWC + 2];
+#pragma unroll
+        for (int q = 0; q < RWC + 2; q++) {
+            const int i = c - 1 + q;
+            const bool ok = i <= N + 1;
+            tz[q] = ok ? t_z[i * W + j] : 0.0;
+            zz[q] = ok ? z[i * W + j] : 0.0;
+            kk[q] = ok ? kh[i * W + j] : 2;
+        }
+#pragma unroll
+        for (int q = 1; q <= RWC; q++) {
+            const int i = c - 1 + q;
+            if (i > N) break;
+            const double tzi = tz[q];
+            if (tzi > zz[q]) {
+                if (kk[q - 1] == 1) {
+                    tz[q - 1] = tzi;
+                    kk[q - 1] = 0;
+                    t_z[(i - 1) * W + j] = tzi;
+                    H_moi[(i - 1) * W + j] = htaiz[(i - 1) * W + j] + tzi;
+                    kh[(i - 1) * W + j] = 0;
+                    FS[(i - 1) * W + j] = FS[i * W + j];
+                }
+                if (kk[q + 1] == 1) {
+                    tz[q + 1] = tzi;
+                    kk[q + 1] = 0;
+                    t_z[(i + 1) * W + j] = tzi;
+                    H_moi[(i + 1) * W + j] = htaiz[(i + 1) * W + j] + tzi;
+                    kh[(i + 1) * W + j] = 0;
+                    FS[(i + 1) * W + j] = FS[i * W + j];
+                }
             }
         }
     }

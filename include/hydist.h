@@ -112,6 +112,7 @@ typedef struct hd_config {
     int bed_change;    /* accumulate dH (Calculate_Qb + BedLoad) once t >= bed_change_start */
     int quirks;        /* HD_QUIRK_* bits */
     int device;        /* HIP device ordinal */
+    int dense_layout;  /* 1: device fields use the reference's width M+3 (no row padding) */
 } hd_config;
 
 /* Field ids for hd_copy_out / hd_copy_in. */
