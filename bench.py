@@ -76,7 +76,13 @@ def stage_bytes(name, sim, sc):
         "k_update_fs": 16,
         "k_qb": 7 * 8 + 4 + 16,
         "k_bedload": 12 * 8 + 4 + 16,
+        "k_segment_rows": 4,          # khouot in (segment tables are O(lines))
+        "k_segment_cols": 4,
+        "k_dry": 8 + 4,
     }
+    if name.startswith("k_rewet"):
+        # candidate mask words in (the marked rows' reads are data dependent)
+        return 8.0 * ((sc.N >> 6) + 1) * sc.M
     for k, v in per_cell.items():
         if name.startswith(k):
             return float(v) * cells
@@ -141,10 +147,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--forms", default="plain", help="comma list of plain,fast,fp32")
+    ap.add_argument("--forms", default="plain,fast,fp32", help="comma list of plain,fast,fp32")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sediment-from-start", action="store_true",
+                    help="switch sediment and bed change on at t=0 (short profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has a single MI355X, and this build "
@@ -156,6 +164,8 @@ def main():
     names = [f for f in args.forms.split(",") if f]
     log("[bench] generating the 2048x1024 meander + islands mesh")
     sc = S.config2()
+    if args.sediment_from_start:
+        sc.sediment_start = sc.bed_change_start = 0.0
     results = {}
     best = None
     for name in names:
@@ -181,6 +191,15 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                     "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms}
+    # the tridiagonal solve kernel of the best form, at its own roofline
+    tri = None
+    solver = [(k, v) for k, v in stats.items() if k.startswith(("k_cr", "k_thomas")) and "fs" not in k]
+    if solver:
+        k, (launches, total_ms) = max(solver, key=lambda kv: kv[1][1])
+        b = stage_bytes(k, sim, sc)
+        a = b / (total_ms / launches * 1e-3) / 1e9
+        tri = {"kernel": k, "bytes_per_launch": b, "avg_launch_ms": total_ms / launches, "achieved": a,
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS}
     step_bytes = STEP_BYTES["bed_change"] * sc.M * sc.N
     step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9}
     step_roof["frac"] = step_roof["achieved_GBs"] / HBM_PEAK_GBS
@@ -211,6 +230,7 @@ def main():
                   for k, v in results.items()},
         "roofline": roofline,
         "step_roofline": step_roof,
+        "tridiag_roofline": tri,
         "cpu_baseline": cpu,
         "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / v[0]} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],

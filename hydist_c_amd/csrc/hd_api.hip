@@ -154,7 +154,7 @@ int prof_resolve(hd_ctx *ctx) {
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n) {
     hipStream_t st = ctx->stream;
     LAUNCH(ctx, "k_segment_rows", k_segment_rows<<<dim3((s.N - 1 + 3) / 4, n), 256, 0, st>>>(S));
-    LAUNCH(ctx, "k_segment_cols", k_segment_cols<<<dim3((s.M - 1 + 63) / 64, n), 64, 0, st>>>(S));
+    LAUNCH(ctx, "k_segment_cols", k_segment_cols<<<dim3((s.M - 1 + SEG_COLS - 1) / SEG_COLS, n), SEG_COLS * SEG_GROUPS, 0, st>>>(S));
     LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
     return HD_OK;
 }

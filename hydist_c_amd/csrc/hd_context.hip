@@ -84,6 +84,7 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
     A((void **)&s.Ap, scratch * sizeof(double));
     A((void **)&s.Bp, scratch * sizeof(double));
     A((void **)&s.acc, (8 + (size_t)s.H) * sizeof(double));
+    A((void **)&s.rwmask, (size_t)((s.N >> 6) + 1) * s.P * sizeof(unsigned long long));
     // line-sweep buffers: ROW_MAJOR [rho][line] and LINE_MAJOR [line][rho]
     SweepBufs &B = ctx->sweep;
     const int rowsU = 2 * (s.N - 1), rowsV = 2 * (s.M - 1);

@@ -44,6 +44,9 @@ struct DevState {
     double *tmp;       // smoothing output (full field)
     double *Ap, *Bp;   // Thomas sweep scratch, two rows per cell
     double *acc;       // [0] boundary inflow volume, [1] mass-balance result
+    // re-wetting candidates: bit (i & 63) of word [(i >> 6) * P + j] marks
+    // row i of column j; cleared by the re-wetting pass that reads it
+    unsigned long long *rwmask;
 };
 
 HD_DEV int at(const DevState &s, int i, int j) { return i * s.P + j; }

@@ -7,6 +7,8 @@ namespace hd {
 
 __global__ void k_onetime_init(const DevState *S);
 __global__ void k_segment_rows(const DevState *S);
+// k_segment_cols: a workgroup of SEG_COLS x SEG_GROUPS threads per SEG_COLS columns
+constexpr int SEG_COLS = 32, SEG_GROUPS = 32;
 __global__ void k_segment_cols(const DevState *S);
 __global__ void k_htuongdoi(const DevState *S);
 __global__ void k_preprocess(const DevState *S);

@@ -81,45 +81,63 @@ __global__ void __launch_bounds__(256) k_segment_rows(const DevState *__restrict
     if (lane == 0) s.segH_n[i] = n;
 }
 
-// Columns (lines along i, stride P): one lane per column, lanes of a
-// wavefront read adjacent columns of the same row.  Each lane loads a chunk
-// of SEGC rows with independent loads before scanning it, so the memory
-// latency is paid once per chunk.
-constexpr int SEGC = 32;
-
-__global__ void __launch_bounds__(64) k_segment_cols(const DevState *__restrict__ S) {
+// Columns (lines along i, stride P): a workgroup takes SEG_COLS adjacent
+// columns (one cache line of khouot per row) and splits the rows 2..N into
+// SEG_GROUPS contiguous groups, one thread per (group, column).  Pass 1
+// counts run starts per group, an LDS prefix gives each group its first
+// segment index, pass 2 writes the run starts and ends.
+__global__ void __launch_bounds__(SEG_COLS *SEG_GROUPS) k_segment_cols(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.y];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    if (j > s.M) return;
+    __shared__ int cnt[SEG_GROUPS][SEG_COLS];
+    __shared__ int lastst[SEG_GROUPS][SEG_COLS];
+    const int tx = threadIdx.x % SEG_COLS, g = threadIdx.x / SEG_COLS;
+    const int j = blockIdx.x * SEG_COLS + tx + 2;
     const int end = s.N, W = s.P;
+    const int rows = end - 1;  // rows 2..end
+    const int per = (rows + SEG_GROUPS - 1) / SEG_GROUPS;
+    const int r0 = 2 + g * per, r1 = min(end + 1, r0 + per);  // [r0, r1)
+    const bool live = j <= s.M;
     const int *kh = s.khouot;
-    int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
-    int n = 0;
-    bool prev_wet = false;
-    int cur = kh[2 * W + j];
-    for (int c = 2; c <= end; c += SEGC) {
-        int nxt[SEGC];
-#pragma unroll
-        for (int q = 0; q < SEGC; q++) {
-            const int p = c + 1 + q;  // row p+1 of the scan, read ahead
-            nxt[q] = p <= end ? kh[p * W + j] : 2;
-        }
-#pragma unroll
-        for (int q = 0; q < SEGC; q++) {
-            const int p = c + q;
-            if (p > end) break;
-            const bool wet = cur == 0;
-            if (wet && !prev_wet) first[n] = p;
-            if (wet && (p == end || nxt[q] != 0)) {
-                last[n] = p;
+    int n = 0, ls = -1;
+    if (live && r0 < r1) {
+        bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
+        for (int p = r0; p < r1; p++) {
+            const bool wet = kh[p * W + j] == 0;
+            if (wet && !prev) {
                 n++;
+                ls = p;
             }
-            prev_wet = wet;
-            cur = nxt[q];
+            prev = wet;
         }
     }
-    if (n > 0 && first[n - 1] >= end - 1) n--;
-    s.segV_n[j] = n;
+    cnt[g][tx] = n;
+    lastst[g][tx] = ls;
+    __syncthreads();
+    int off = 0;
+    for (int q = 0; q < g; q++) off += cnt[q][tx];
+    if (live && r0 < r1) {
+        int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
+        int k = off;  // starts at rows <= p
+        bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
+        bool wet = kh[r0 * W + j] == 0;
+        for (int p = r0; p < r1; p++) {
+            const bool nwet = p < end && kh[(p + 1) * W + j] == 0;
+            if (wet && !prev) first[k++] = p;
+            if (wet && !nwet) last[k - 1] = p;
+            prev = wet;
+            wet = nwet;
+        }
+    }
+    if (g == 0 && live) {
+        int total = 0, lst = -1;
+        for (int q = 0; q < SEG_GROUPS; q++) {
+            total += cnt[q][tx];
+            lst = max(lst, lastst[q][tx]);
+        }
+        // a run starting at end-1 or end is not a segment; it is always the last run
+        if (total > 0 && lst >= end - 1) total--;
+        s.segV_n[j] = total;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -427,7 +445,9 @@ __global__ void k_post_a(const DevState *__restrict__ S) {
 }
 
 // Second half-step before Reset_states_vertical (engine.cpp:229-236):
-// update_buffer(u) and update_h_moi.
+// update_buffer(u) and update_h_moi.  Also marks the rows at which the
+// re-wetting scan can act: t_z > z with a dry (khouot == 1) neighbour in
+// the column, from the values the scan starts with.
 __global__ void k_pre_reset(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,58 +456,58 @@ __global__ void k_pre_reset(const DevState *__restrict__ S) {
     const int W = s.P, pos = i * W + j;
     if (interior(s, i, j)) s.t_u[pos] = s.tmp[pos];
     if (i >= 1 && j >= 1 && s.khouot[pos] == 0) s.H_moi[pos] = s.htaiz[pos] + s.t_z[pos];
+    if (i >= 2 && j >= 1 && s.t_z[pos] > s.z[pos] && (s.khouot[pos - W] == 1 || s.khouot[pos + W] == 1))
+        atomicOr(&s.rwmask[(size_t)(i >> 6) * W + j], 1ull << (i & 63));
 }
 
 // Reset_states_vertical (support_funcs.cpp:142-188), re-wetting pass: one
-// lane per column j in [1, M], serial along i (lanes read adjacent
-// columns).  t_z, z and khouot of a chunk of rows are read ahead into
-// registers; every update is written through to memory at once, so the
-// next chunk's loads and the rare H_moi / FS reads see it.
-constexpr int RWC = 32;
+// lane per column j in [1, M].  The reference scans i = 2..N in order; a
+// row changes anything only if it is a marked row (see k_pre_reset) or the
+// row below one whose scan re-wetted it, so the lane visits the marked rows
+// in ascending order and follows each cascade downward while it lasts.
+// Every update is written through, so later rows read current values.
+HD_DEV bool rewet_row(const DevState &s, int i, int j) {
+    const int W = s.P;
+    double *t_z = s.t_z, *H_moi = s.H_moi, *FS = s.FS;
+    int *kh = s.khouot;
+    const double tzi = t_z[i * W + j];
+    bool down = false;
+    if (tzi > s.z[i * W + j]) {
+        if (kh[(i - 1) * W + j] == 1) {
+            t_z[(i - 1) * W + j] = tzi;
+            H_moi[(i - 1) * W + j] = s.htaiz[(i - 1) * W + j] + tzi;
+            kh[(i - 1) * W + j] = 0;
+            FS[(i - 1) * W + j] = FS[i * W + j];
+        }
+        if (kh[(i + 1) * W + j] == 1) {
+            t_z[(i + 1) * W + j] = tzi;
+            H_moi[(i + 1) * W + j] = s.htaiz[(i + 1) * W + j] + tzi;
+            kh[(i + 1) * W + j] = 0;
+            FS[(i + 1) * W + j] = FS[i * W + j];
+            down = true;
+        }
+    }
+    return down;
+}
 
 __global__ void __launch_bounds__(64) k_rewet(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.y];
     const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (j > s.M) return;
-    const int W = s.P, N = s.N;
-    double *t_z = s.t_z, *H_moi = s.H_moi, *FS = s.FS;
-    const double *z = s.z, *htaiz = s.htaiz;
-    int *kh = s.khouot;
-    for (int c = 2; c <= N; c += RWC) {
-        // window rows c-1 .. c+RWC
-        double tz[RWC + 2], zz[RWC + 2];
-        int kk[RWC + 2];
-#pragma unroll
-        for (int q = 0; q < RWC + 2; q++) {
-            const int i = c - 1 + q;
-            const bool ok = i <= N + 1;
-            tz[q] = ok ? t_z[i * W + j] : 0.0;
-            zz[q] = ok ? z[i * W + j] : 0.0;
-            kk[q] = ok ? kh[i * W + j] : 2;
-        }
-#pragma unroll
-        for (int q = 1; q <= RWC; q++) {
-            const int i = c - 1 + q;
-            if (i > N) break;
-            const double tzi = tz[q];
-            if (tzi > zz[q]) {
-                if (kk[q - 1] == 1) {
-                    tz[q - 1] = tzi;
-                    kk[q - 1] = 0;
-                    t_z[(i - 1) * W + j] = tzi;
-                    H_moi[(i - 1) * W + j] = htaiz[(i - 1) * W + j] + tzi;
-                    kh[(i - 1) * W + j] = 0;
-                    FS[(i - 1) * W + j] = FS[i * W + j];
-                }
-                if (kk[q + 1] == 1) {
-                    tz[q + 1] = tzi;
-                    kk[q + 1] = 0;
-                    t_z[(i + 1) * W + j] = tzi;
-                    H_moi[(i + 1) * W + j] = htaiz[(i + 1) * W + j] + tzi;
-                    kh[(i + 1) * W + j] = 0;
-                    FS[(i + 1) * W + j] = FS[i * W + j];
-                }
-            }
+    const int W = s.P, N = s.N, nw = (N >> 6) + 1;
+    int next = 2;  // first row not yet scanned
+    for (int w = 0; w < nw; w++) {
+        unsigned long long m = s.rwmask[(size_t)w * W + j];
+        if (!m) continue;
+        s.rwmask[(size_t)w * W + j] = 0;
+        while (m) {
+            const int i = w * 64 + __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            if (i < next || i > N) continue;
+            int r = i;
+            // a row re-wetted from above is scanned next with its new level
+            while (rewet_row(s, r, j) && r < N) r++;
+            next = r + 1;
         }
     }
 }
