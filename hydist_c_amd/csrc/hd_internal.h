@@ -62,6 +62,14 @@ int prof_resolve(hd_ctx *ctx);
         if (rc_) return rc_;                            \
         hd::prof_end((ctx), (what), e0_);               \
     } while (0)
+// As LAUNCH, storing the status in `rc` instead of returning it.
+#define LAUNCH_RC(ctx, rc, what, ...)                   \
+    do {                                                \
+        hipEvent_t e0_ = hd::prof_begin(ctx);           \
+        __VA_ARGS__;                                    \
+        (rc) = hd::check_launch((ctx), (what));         \
+        if (!(rc)) hd::prof_end((ctx), (what), e0_);    \
+    } while (0)
 int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh);
 
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);

@@ -554,10 +554,14 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
     return o;
 }
 
-template <typename T, bool REFINE, bool INTERLEAVED>
+// One workgroup per line; thread t owns rows q = t + k * blockDim.x,
+// k < KPT.  The FP64 rows a thread needs again (its velocity row for the
+// recovery, its reduced z row for the residual) stay in its registers, so
+// each coefficient is read from HBM once and each unknown written once.
+template <typename T, bool REFINE, bool INTERLEAVED, int KPT>
 __global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs B, int start, int end,
                                            int nrows) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ double smem[];
     const int line = start + blockIdx.x;
     if (line > end) return;
     const int ld = B.ld;
@@ -566,66 +570,99 @@ __global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, Swee
     const double *__restrict__ C = B.C + (size_t)line * ld;
     const double *__restrict__ D = B.D + (size_t)line * ld;
     double *__restrict__ X = B.X + (size_t)line * ld;
-    // FP64 reduced system kept for the residual (global scratch Ap/Bp: a,b | c,d)
-    double *__restrict__ Ra = B.Ap + (size_t)line * ld;
-    double *__restrict__ Rb = Ra + ld / 2;
-    double *__restrict__ Rc = B.Bp + (size_t)line * ld;
-    double *__restrict__ Rd = Rc + ld / 2;
     const int n = INTERLEAVED ? nrows / 2 : nrows;
     T *base = reinterpret_cast<T *>(smem);
     CRShared<T> sh{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n};
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < n; q += nt) {
+    Row red[KPT], vel[KPT];
+    double zf[KPT];
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+        const int q = tid + k * nt;
+        if (q >= n) continue;
         Row r;
-        if (INTERLEAVED)
+        if (INTERLEAVED) {
             r = reduce_z_row(A, Bv, C, D, q, nrows);
-        else
+            const int rv = 2 * q + 1;
+            vel[k] = rv < nrows ? Row{A[rv], Bv[rv], C[rv], D[rv]} : Row{0.0, 1.0, 0.0, 0.0};
+        } else {
             r = Row{A[q], Bv[q], C[q], D[q]};
+        }
         sh.a[q] = (T)r.a;
         sh.b[q] = (T)r.b;
         sh.c[q] = (T)r.c;
         sh.d[q] = (T)r.d;
-        if (REFINE) {
-            Ra[q] = r.a;
-            Rb[q] = r.b;
-            Rc[q] = r.c;
-            Rd[q] = r.d;
-        }
+        red[k] = r;
     }
     __syncthreads();
     cr_solve<T>(sh, n, REFINE);
     if (REFINE) {
-        // z0 -> X (FP64), residual of the FP64 system -> d
-        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+        // residual of the FP64 system at the T solution, solved again with
+        // the stored multipliers
+        double rr[KPT];
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q >= n) continue;
+            const double xq = (double)sh.d[q];
+            zf[k] = xq;
+            double v = red[k].d - red[k].b * xq;
+            if (q > 0) v = v - red[k].a * (double)sh.d[q - 1];
+            if (q + 1 < n) v = v - red[k].c * (double)sh.d[q + 1];
+            rr[k] = v;
+        }
         __syncthreads();
-        for (int q = tid; q < n; q += nt) {
-            const int stp = INTERLEAVED ? 2 : 1;
-            const double xq = X[stp * q];
-            double rr = Rd[q] - Rb[q] * xq;
-            if (q > 0) rr = rr - Ra[q] * X[stp * (q - 1)];
-            if (q + 1 < n) rr = rr - Rc[q] * X[stp * (q + 1)];
-            sh.d[q] = (T)rr;
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q < n) sh.d[q] = (T)rr[k];
         }
         __syncthreads();
         cr_replay<T>(sh, n);
-        for (int q = tid; q < n; q += nt) {
-            const int stp = INTERLEAVED ? 2 : 1;
-            X[stp * q] = X[stp * q] + (double)sh.d[q];
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q < n) zf[k] = zf[k] + (double)sh.d[q];
         }
     } else {
-        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q < n) zf[k] = (double)sh.d[q];
+        }
     }
-    if (INTERLEAVED) {
+    if (!INTERLEAVED) {
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q < n) X[q] = zf[k];
+        }
+        return;
+    }
+    // FP64 z of the line in LDS (over a/b, free after the solve; with T =
+    // double sh.d already holds it) for the neighbours' velocity recovery
+    const double *zs = reinterpret_cast<const double *>(sh.d);
+    if (REFINE) {
+        double *zw = smem;
+#pragma unroll
+        for (int k = 0; k < KPT; k++) {
+            const int q = tid + k * nt;
+            if (q < n) zw[q] = zf[k];
+        }
+        zs = zw;
         __syncthreads();
-        // velocity rows from their own FP64 rows
-        for (int q = tid; q < n; q += nt) {
-            const int r = 2 * q + 1;
-            if (r >= nrows) continue;
-            double v = D[r];
-            const double a = A[r], c = C[r];
-            if (a != 0.0) v = v - a * X[r - 1];
-            if (c != 0.0 && r + 1 < nrows) v = v - c * X[r + 1];
-            X[r] = v / Bv[r];
+    }
+#pragma unroll
+    for (int k = 0; k < KPT; k++) {
+        const int q = tid + k * nt;
+        if (q >= n) continue;
+        const int r = 2 * q + 1;
+        X[2 * q] = zf[k];
+        if (r < nrows) {
+            double v = vel[k].d;
+            if (vel[k].a != 0.0) v = v - vel[k].a * zf[k];
+            if (vel[k].c != 0.0 && r + 1 < nrows) v = v - vel[k].c * zs[q + 1];
+            X[r] = v / vel[k].b;
         }
     }
 }
@@ -766,30 +803,35 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
     } else {
         const int nz = hyd ? nrows / 2 : nrows;
+        if (n != 1) return HD_E_UNSUPPORTED;
         const int threads = nz > 1024 ? 512 : 256;
-        if (form == HD_FORM_FP32_REFINE) {
-            const size_t sm = 6 * (size_t)nz * sizeof(float);
-            if (n != 1) return HD_E_UNSUPPORTED;
-            int rc = allow_lds(ctx, hyd ? k_cr<float, true, true> : k_cr<float, true, false>, sm);
-            if (rc) return rc;
-            if (hyd)
-                LAUNCH(ctx, DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>",
-                       (k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
-            else
-                LAUNCH(ctx, DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>",
-                       (k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
-        } else {
-            const size_t sm = 4 * (size_t)nz * sizeof(double);
-            if (n != 1) return HD_E_UNSUPPORTED;
-            int rc = allow_lds(ctx, hyd ? k_cr<double, false, true> : k_cr<double, false, false>, sm);
-            if (rc) return rc;
-            if (hyd)
-                LAUNCH(ctx, DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>",
-                       (k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
-            else
-                LAUNCH(ctx, DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>",
-                       (k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+        const int kpt = (nz + threads - 1) / threads;
+        if (kpt > 8) {
+            ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the solver's 4096";
+            return HD_E_UNSUPPORTED;
         }
+        const bool f32 = form == HD_FORM_FP32_REFINE;
+        const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
+        const char *name = f32 ? (hyd ? (DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>") : (DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>"))
+                               : (hyd ? (DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>") : (DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>"));
+        int rc = HD_OK;
+        auto go = [&](auto kern) {
+            rc = allow_lds(ctx, kern, sm);
+            if (rc) return;
+            LAUNCH_RC(ctx, rc, name, (kern<<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+        };
+        auto pick = [&](auto kp) {
+            constexpr int K = decltype(kp)::value;
+            if (f32) {
+                if (hyd) go(k_cr<float, true, true, K>); else go(k_cr<float, true, false, K>);
+            } else {
+                if (hyd) go(k_cr<double, false, true, K>); else go(k_cr<double, false, false, K>);
+            }
+        };
+        if (kpt <= 2) pick(std::integral_constant<int, 2>{});
+        else if (kpt <= 4) pick(std::integral_constant<int, 4>{});
+        else pick(std::integral_constant<int, 8>{});
+        if (rc) return rc;
     }
     const Launch2D g = grid2d(s, n);
     if (hyd)
