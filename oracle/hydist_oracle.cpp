@@ -15,6 +15,19 @@
  * here are stated at the place they apply ("Define:").  Deterministic
  * quirks are kept ("Keep:").
  *
+ * What pins each part (tests/test_oracle.py): the khouot dump that ships
+ * with the reference (Onetime_init), Load_coeffs' constants, dense and
+ * LAPACK-banded solves (tridiag), brute-force run-length encoding and the
+ * edge rules (segmentation), closed forms (boundary interpolation,
+ * source_chs/source_nchs), and whole-trajectory invariants (lake at rest
+ * stays exactly 0, closed-basin volume, inflow balance, uniform FS stays
+ * uniform, assembled systems are solved).
+ * Parity unpinned: the friction / advection / viscosity coefficients of
+ * the momentum rows (_calculate_abcd, uSolver, vSolver), the smoothing
+ * weights (_normalize), Find_VTH / hesoK, and the bed-change terms
+ * (Calculate_Qb, _bed_load; never launched by the reference, restated in
+ * their intended form) have no check beyond those invariants.
+ *
  * Parallelism: OpenMP over independent lines (sweeps) and rows (stencils).
  * Every parallel loop writes disjoint data, so results do not depend on
  * the thread count.
