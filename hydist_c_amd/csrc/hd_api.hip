@@ -54,8 +54,9 @@ int ctx_alloc(hd_ctx *ctx, void **p, size_t bytes) {
     return HD_OK;
 }
 
-double *field_ptr(hd_ctx *ctx, int f) {
-    DevState &s = ctx->hs;
+double *field_ptr(hd_ctx *ctx, int f) { return field_ptr(ctx->hs, f); }
+
+double *field_ptr(const DevState &s, int f) {
     switch (f) {
         case HD_F_U: return s.u;
         case HD_F_V: return s.v;
@@ -94,8 +95,30 @@ Launch2D grid2d(const DevState &s, int nscen) {
 }
 
 // Pushes the host descriptor to the device copy.
+// Pushes the host descriptors to the device copy.  Scenario 0 (hs) holds
+// the parameters every scenario shares (coefficients, quirks, cohesion).
 int sync_desc(hd_ctx *ctx) {
+    if (!ctx->ds) return HD_OK;
+    for (DevState &d : ctx->hss) {
+        d.c = ctx->hs.c;
+        d.quirks = ctx->hs.quirks;
+        d.ketdinh = ctx->hs.ketdinh;
+        d.clock = ctx->hs.clock;
+    }
     HD_HIP(ctx, hipMemcpyAsync(ctx->ds, &ctx->hs, sizeof(DevState), hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->hss.empty())
+        HD_HIP(ctx, hipMemcpyAsync(ctx->ds + 1, ctx->hss.data(), ctx->hss.size() * sizeof(DevState),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    return HD_OK;
+}
+
+// Sets the device clock to t, or (while a step is being captured into a
+// graph, which replays with later times) advances it by half a step.
+int set_clock(hd_ctx *ctx, double t) {
+    if (ctx->capturing)
+        LAUNCH(ctx, "k_clock", k_clock<<<1, 64, 0, ctx->stream>>>(ctx->clock, 0.5 * ctx->c.dT, 1));
+    else
+        LAUNCH(ctx, "k_clock", k_clock<<<1, 64, 0, ctx->stream>>>(ctx->clock, t, 0));
     return HD_OK;
 }
 
@@ -174,13 +197,15 @@ int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     hipStream_t st = ctx->stream;
     const hd_options &ops = ctx->ops;
     Launch2D g = grid2d(s, n);
-    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S, t));
+    int rc = set_clock(ctx, t);
+    if (rc) return rc;
+    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S));
     int start = 2, end = s.M;
     if (ops.channel && ops.kenhhepd) {
         start = 3;
         end = s.M - 1;
     }
-    int rc = run_sweep(ctx, S, s, n, 1, start, end);
+    rc = run_sweep(ctx, S, s, n, 1, start, end);
     if (rc) return rc;
     LAUNCH(ctx, "k_smooth<u>", k_smooth<true><<<g.grid, g.block, 0, st>>>(S));
     LAUNCH(ctx, "k_solve_v", k_solve_v<<<g.grid, g.block, 0, st>>>(S));
@@ -192,7 +217,7 @@ int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
     if (ctx->cfg.sediment && t >= ops.sediment_start) {
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
-        rc = run_line_sweep(ctx, S, s, n, 1, false, 3, s.M - 1, t, ops.bed_change_start);
+        rc = run_line_sweep(ctx, S, s, n, 1, false, 3, s.M - 1, ops.bed_change_start);
         if (rc) return rc;
         LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
     }
@@ -203,10 +228,12 @@ int run_halfstep_vz_a(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     hipStream_t st = ctx->stream;
     const hd_options &ops = ctx->ops;
     Launch2D g = grid2d(s, n);
-    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S, t));
+    int rc = set_clock(ctx, t);
+    if (rc) return rc;
+    if (!ops.channel) LAUNCH(ctx, "k_boundary", k_boundary<<<n, 1024, 0, st>>>(S));
     int start = 2, end = s.N;
     if (ops.channel && ops.kenhhepng) start = 3;
-    int rc = run_sweep(ctx, S, s, n, 0, start, end);
+    rc = run_sweep(ctx, S, s, n, 0, start, end);
     if (rc) return rc;
     LAUNCH(ctx, "k_smooth<v>", k_smooth<false><<<g.grid, g.block, 0, st>>>(S));
     LAUNCH(ctx, "k_solve_u", k_solve_u<<<g.grid, g.block, 0, st>>>(S));
@@ -229,7 +256,7 @@ int run_halfstep_vz_b(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     if (rc) return rc;
     if (ctx->cfg.sediment && t >= ops.sediment_start) {
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
-        rc = run_line_sweep(ctx, S, s, n, 0, false, 3, s.N - 1, t, ops.bed_change_start);
+        rc = run_line_sweep(ctx, S, s, n, 0, false, 3, s.N - 1, ops.bed_change_start);
         if (rc) return rc;
         LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
         if (ctx->cfg.bed_change && t >= ops.bed_change_start) {
@@ -247,7 +274,7 @@ int run_halfstep_vz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
 }
 
 int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, int start, int end) {
-    return run_line_sweep(ctx, S, s, n, dir, true, start, end, 0.0, 0.0);
+    return run_line_sweep(ctx, S, s, n, dir, true, start, end, 0.0);
 }
 
 }  // namespace hd
@@ -312,16 +339,28 @@ int hd_load_coeffs(hd_coeffs *ce) {
     return hd_finalize_coeffs(ce);
 }
 
-int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops, const hd_config *cfg,
-              hd_ctx **out) {
-    if (!out) return set_err(nullptr, HD_E_INVALID, "out is NULL");
-    *out = nullptr;
-    if (!mesh || !coeffs || !ops) return set_err(nullptr, HD_E_INVALID, "mesh/coeffs/ops is NULL");
+static int check_mesh(const hd_mesh *mesh) {
     if (mesh->M < 4 || mesh->N < 4 || !mesh->h || !mesh->hsnham || !mesh->VISCOIDX)
         return set_err(nullptr, HD_E_INVALID, "mesh needs M, N >= 4 and h, hsnham, VISCOIDX");
     if (mesh->total_time < 2) return set_err(nullptr, HD_E_INVALID, "boundary series need >= 2 samples");
     for (int s = 0; s < 4; s++)
         if (!mesh->bc[s]) return set_err(nullptr, HD_E_INVALID, "boundary series bc[side] is NULL");
+    return HD_OK;
+}
+
+// One context for n scenarios of one shape (n = 1: hd_create).
+static int create_ctx(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, const hd_options *ops,
+                      const hd_config *cfg, hd_ctx **out) {
+    if (!out) return set_err(nullptr, HD_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!meshes || !coeffs || !ops || n < 1) return set_err(nullptr, HD_E_INVALID, "mesh/coeffs/ops is NULL");
+    for (int k = 0; k < n; k++) {
+        int rc = check_mesh(&meshes[k]);
+        if (rc) return rc;
+        if (meshes[k].M != meshes[0].M || meshes[k].N != meshes[0].N ||
+            meshes[k].total_time != meshes[0].total_time)
+            return set_err(nullptr, HD_E_INVALID, "batched scenarios need one M, N and total_time");
+    }
     const int form = cfg ? cfg->form : HD_FORM_FP64_PLAIN;
     if (form < HD_FORM_FP64_PLAIN || form > HD_FORM_FP32_REFINE)
         return set_err(nullptr, HD_E_INVALID, "unknown form");
@@ -335,8 +374,8 @@ int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *op
     ctx->device = dev;
     ctx->c = *coeffs;
     ctx->ops = *ops;
-    ctx->ops.M = mesh->M;
-    ctx->ops.N = mesh->N;
+    ctx->ops.M = meshes[0].M;
+    ctx->ops.N = meshes[0].N;
     if (cfg) ctx->cfg = *cfg;
     ctx->cfg.form = form;
     if (hipSetDevice(dev) != hipSuccess) {
@@ -347,7 +386,14 @@ int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *op
         delete ctx;
         return set_err(nullptr, HD_E_HIP, "hipStreamCreate failed");
     }
-    int rc = hd::alloc_scenario(ctx, mesh);
+    int rc = hd::alloc_scenario(ctx, &meshes[0], ctx->hs);
+    for (int k = 1; k < n && !rc; k++) {
+        DevState d{};
+        rc = hd::alloc_scenario(ctx, &meshes[k], d);
+        if (!rc) ctx->hss.push_back(d);
+    }
+    if (!rc) rc = hd::alloc_shared(ctx, n);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_err(ctx, HD_E_HIP, "upload failed");
     if (rc) {
         g_err = ctx->err;
         hd_destroy(ctx);
@@ -357,10 +403,15 @@ int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *op
     return HD_OK;
 }
 
+int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops, const hd_config *cfg,
+              hd_ctx **out) {
+    return create_ctx(mesh, 1, coeffs, ops, cfg, out);
+}
+
 int hd_init(hd_ctx *ctx) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     HD_HIP(ctx, hipSetDevice(ctx->device));
-    int rc = run_init(ctx, ctx->ds, ctx->hs, 1);
+    int rc = run_init(ctx, ctx->ds, ctx->hs, ctx->nscen);
     if (rc) return rc;
     ctx->inited = true;
     return HD_OK;
@@ -368,7 +419,9 @@ int hd_init(hd_ctx *ctx) {
 
 int hd_update_boundary(hd_ctx *ctx, double t) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
-    LAUNCH(ctx, "k_boundary", k_boundary<<<1, 1024, 0, ctx->stream>>>(ctx->ds, t));
+    int rc = set_clock(ctx, t);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_boundary", k_boundary<<<ctx->nscen, 1024, 0, ctx->stream>>>(ctx->ds));
     return HD_OK;
 }
 
@@ -376,24 +429,78 @@ int hd_halfstep_uz(hd_ctx *ctx, double t) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
     if (ctx->fault_stage == 1) return set_err(ctx, HD_E_FAULT, "injected fault in UZ half-step");
-    return run_halfstep_uz(ctx, ctx->ds, ctx->hs, 1, t);
+    return run_halfstep_uz(ctx, ctx->ds, ctx->hs, ctx->nscen, t);
 }
 
 int hd_halfstep_vz(hd_ctx *ctx, double t) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
     if (ctx->fault_stage == 2) return set_err(ctx, HD_E_FAULT, "injected fault in VZ half-step");
-    return run_halfstep_vz(ctx, ctx->ds, ctx->hs, 1, t);
+    return run_halfstep_vz(ctx, ctx->ds, ctx->hs, ctx->nscen, t);
 }
 
-int hd_step(hd_ctx *ctx, double *t) {
-    if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
+// Which optional stages a step from t runs: sediment in the first and the
+// second half-step, bed change in the second.
+static int step_key(const hd_ctx *ctx, double t) {
+    const double dT = ctx->c.dT, t1 = t + 0.5 * dT, t2 = t1 + dT * 0.5;
+    const hd_options &o = ctx->ops;
+    const bool sed = ctx->cfg.sediment != 0;
+    return (sed && t1 >= o.sediment_start ? 1 : 0) | (sed && t2 >= o.sediment_start ? 2 : 0) |
+           (sed && ctx->cfg.bed_change && t2 >= o.sediment_start && t2 >= o.bed_change_start ? 4 : 0);
+}
+
+static int step_eager(hd_ctx *ctx, double *t) {
     const double dT = ctx->c.dT;
     *t += 0.5 * dT;
     int rc = hd_halfstep_uz(ctx, *t);
     if (rc) return rc;
     *t += dT * 0.5;
     return hd_halfstep_vz(ctx, *t);
+}
+
+// A whole step as one graph launch.  The graph for a stage pattern is
+// captured once; it advances the device clock itself, so a replay needs
+// only the clock set to the step's start time.
+static int step_graph(hd_ctx *ctx, double *t) {
+    const int key = step_key(ctx, *t);
+    auto it = ctx->graphs.find(key);
+    if (it == ctx->graphs.end()) {
+        if (ctx->prof) {
+            int rc = prof_resolve(ctx);
+            if (rc) return rc;
+        }
+        const bool prof = ctx->prof;
+        ctx->prof = false;
+        HD_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        ctx->capturing = true;
+        double tt = *t;
+        int rc = step_eager(ctx, &tt);
+        ctx->capturing = false;
+        ctx->prof = prof;
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        hipGraphExec_t x = nullptr;
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+        it = ctx->graphs.emplace(key, x).first;
+    }
+    LAUNCH(ctx, "k_clock", k_clock<<<1, 64, 0, ctx->stream>>>(ctx->clock, *t, 0));
+    HD_HIP(ctx, hipGraphLaunch(it->second, ctx->stream));
+    *t += ctx->c.dT;
+    return HD_OK;
+}
+
+int hd_step(hd_ctx *ctx, double *t) {
+    if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    if (ctx->use_graph && !ctx->fault_stage) return step_graph(ctx, t);
+    return step_eager(ctx, t);
 }
 
 int hd_run(hd_ctx *ctx, double *t, double Tmax) {
@@ -544,6 +651,7 @@ int hd_destroy(hd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     prof_resolve(ctx);
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -592,6 +700,66 @@ int hd_device_info(char *buf, size_t n) {
     if (hipGetDeviceProperties(&p, 0) != hipSuccess) return HD_E_HIP;
     std::snprintf(buf, n, "%s %s CUs=%d L2=%d MB HBM=%.1f GB", p.name, p.gcnArchName, p.multiProcessorCount,
                   p.l2CacheSize >> 20, p.totalGlobalMem / 1e9);
+    return HD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of scenarios and whole-step graphs
+// ---------------------------------------------------------------------------
+int hd_batch_create(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, const hd_options *ops,
+                    const hd_config *cfg, hd_ctx **out) {
+    return create_ctx(meshes, n, coeffs, ops, cfg, out);
+}
+
+int hd_batch_size(hd_ctx *ctx, int *n) {
+    if (!ctx || !n) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    *n = ctx->nscen;
+    return HD_OK;
+}
+
+static const DevState *scen_desc(hd_ctx *ctx, int scen) {
+    if (scen < 0 || scen >= ctx->nscen) return nullptr;
+    return scen == 0 ? &ctx->hs : &ctx->hss[scen - 1];
+}
+
+int hd_batch_copy_out(hd_ctx *ctx, int scen, int field, double *dst, size_t n) {
+    if (!ctx || !dst) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    const DevState *s = scen_desc(ctx, scen);
+    if (!s) return set_err(ctx, HD_E_INVALID, "scenario index out of range");
+    const size_t W = s->M + 3, H = s->N + 3;
+    if (field == HD_F_KHOUOT) {
+        if (n != W * H) return set_err(ctx, HD_E_INVALID, "bad size");
+        std::vector<int32_t> tmp(n);
+        HD_HIP(ctx, hipMemcpy2DAsync(tmp.data(), W * sizeof(int32_t), s->khouot, s->P * sizeof(int32_t),
+                                     W * sizeof(int32_t), H, hipMemcpyDeviceToHost, ctx->stream));
+        HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t q = 0; q < n; q++) dst[q] = tmp[q];
+        return HD_OK;
+    }
+    double *src = field_ptr(*s, field);
+    if (!src || n != W * H) return set_err(ctx, HD_E_INVALID, "bad field id or size");
+    HD_HIP(ctx, hipMemcpy2DAsync(dst, W * sizeof(double), src, s->P * sizeof(double), W * sizeof(double), H,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t n) {
+    if (!ctx || !src) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    const DevState *s = scen_desc(ctx, scen);
+    if (!s) return set_err(ctx, HD_E_INVALID, "scenario index out of range");
+    double *dst = field_ptr(*s, field);
+    const size_t W = s->M + 3, H = s->N + 3;
+    if (!dst || n != W * H) return set_err(ctx, HD_E_INVALID, "bad field id or size");
+    HD_HIP(ctx, hipMemcpy2DAsync(dst, s->P * sizeof(double), src, W * sizeof(double), W * sizeof(double), H,
+                                 hipMemcpyHostToDevice, ctx->stream));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return HD_OK;
+}
+
+int hd_set_graph(hd_ctx *ctx, int on) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    ctx->use_graph = on != 0;
     return HD_OK;
 }
 

@@ -28,8 +28,7 @@ int upload2d(hd_ctx *ctx, double *dst, const double *src, const DevState &s) {
 
 }  // namespace
 
-int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
-    DevState &s = ctx->hs;
+int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     s.M = mesh->M;
     s.N = mesh->N;
     s.P = ctx->cfg.dense_layout ? s.M + 3 : round_up(s.M + 3, 32);
@@ -85,20 +84,6 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
     A((void **)&s.Bp, scratch * sizeof(double));
     A((void **)&s.acc, (8 + (size_t)s.H) * sizeof(double));
     A((void **)&s.rwmask, (size_t)((s.N >> 6) + 1) * s.P * sizeof(unsigned long long));
-    // line-sweep buffers: ROW_MAJOR [rho][line] and LINE_MAJOR [line][rho]
-    SweepBufs &B = ctx->sweep;
-    const int rowsU = 2 * (s.N - 1), rowsV = 2 * (s.M - 1);
-    B.ld_rm[1] = round_up(s.M + 3, 64);
-    B.ld_rm[0] = round_up(s.N + 3, 64);
-    B.ld_lm[1] = round_up(rowsU, 64);
-    B.ld_lm[0] = round_up(rowsV, 64);
-    const size_t sw = std::max(std::max((size_t)rowsU * B.ld_rm[1], (size_t)rowsV * B.ld_rm[0]),
-                               std::max((size_t)(s.M + 3) * B.ld_lm[1], (size_t)(s.N + 3) * B.ld_lm[0]));
-    for (double **q : {&B.A, &B.B, &B.C, &B.D, &B.X, &B.Ap, &B.Bp}) A((void **)q, sw * sizeof(double));
-    const size_t nsys = std::max((size_t)s.capV * (s.M + 3), (size_t)s.capH * (s.N + 3));
-    A((void **)&B.sys0, nsys * sizeof(int));
-    A((void **)&B.sysn, nsys * sizeof(int));
-    A((void **)&ctx->ds, sizeof(DevState));
     if (rc) return rc;
     s.h = h; s.hsnham = hsn; s.visc = vis; s.Fw = fw; s.Tsxw = tx; s.Tsyw = ty;
     if ((rc = upload2d(ctx, h, mesh->h, s))) return rc;
@@ -119,8 +104,39 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh) {
                                      hipMemcpyHostToDevice) != hipSuccess)
             return HD_E_HIP;
     }
-    if (hipMemcpy(ctx->ds, &s, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess) return HD_E_HIP;
     return HD_OK;
+}
+
+// Buffers shared by the scenarios of a context: the line-sweep buffers (one
+// set per scenario, ROW_MAJOR [rho][line] and LINE_MAJOR [line][rho]
+// layouts), the clock and the device descriptor array.
+int alloc_shared(hd_ctx *ctx, int nscen) {
+    const DevState &s = ctx->hs;
+    int rc = 0;
+    auto A = [&](void **p, size_t bytes) {
+        if (!rc) rc = ctx_alloc(ctx, p, bytes);
+    };
+    SweepBufs &B = ctx->sweep;
+    const int rowsU = 2 * (s.N - 1), rowsV = 2 * (s.M - 1);
+    B.ld_rm[1] = round_up(s.M + 3, 64);
+    B.ld_rm[0] = round_up(s.N + 3, 64);
+    B.ld_lm[1] = round_up(rowsU, 64);
+    B.ld_lm[0] = round_up(rowsV, 64);
+    const size_t sw = std::max(std::max((size_t)rowsU * B.ld_rm[1], (size_t)rowsV * B.ld_rm[0]),
+                               std::max((size_t)(s.M + 3) * B.ld_lm[1], (size_t)(s.N + 3) * B.ld_lm[0]));
+    B.stride = (sw + 63) / 64 * 64;
+    for (double **q : {&B.A, &B.B, &B.C, &B.D, &B.X, &B.Ap, &B.Bp}) A((void **)q, B.stride * nscen * sizeof(double));
+    const size_t nsys = std::max((size_t)s.capV * (s.M + 3), (size_t)s.capH * (s.N + 3));
+    B.sstride = nsys;
+    A((void **)&B.sys0, nsys * nscen * sizeof(int));
+    A((void **)&B.sysn, nsys * nscen * sizeof(int));
+    A((void **)&ctx->clock, 64);
+    A((void **)&ctx->ds, sizeof(DevState) * nscen);
+    if (rc) return rc;
+    ctx->nscen = nscen;
+    ctx->hs.clock = ctx->clock;
+    for (DevState &d : ctx->hss) d.clock = ctx->clock;
+    return sync_desc(ctx);
 }
 
 

@@ -47,6 +47,9 @@ struct DevState {
     // re-wetting candidates: bit (i & 63) of word [(i >> 6) * P + j] marks
     // row i of column j; cleared by the re-wetting pass that reads it
     unsigned long long *rwmask;
+    // simulation time of the current half-step (one value shared by every
+    // scenario of a context), set or advanced by k_clock
+    double *clock;
 };
 
 HD_DEV int at(const DevState &s, int i, int j) { return i * s.P + j; }

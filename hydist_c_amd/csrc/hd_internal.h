@@ -18,8 +18,15 @@ struct hd_ctx {
     hd_coeffs c{};
     hd_options ops{};
     hd_config cfg{};
-    hd::DevState hs{};             // descriptor (host copy)
-    hd::DevState *ds = nullptr;    // descriptor (device copy)
+    hd::DevState hs{};             // descriptor of scenario 0 (host copy)
+    std::vector<hd::DevState> hss; // descriptors of scenarios 1..n-1 of a batch
+    int nscen = 1;
+    hd::DevState *ds = nullptr;    // descriptors of all scenarios (device copy)
+    double *clock = nullptr;       // device time of the current half-step
+    // whole-step graphs, keyed by which optional stages the step runs
+    bool use_graph = false;
+    bool capturing = false;
+    std::map<int, hipGraphExec_t> graphs;
     hd::SweepBufs sweep{};         // line-sweep buffers
     std::vector<void *> allocs;
     size_t bytes = 0;
@@ -45,6 +52,7 @@ struct Launch2D {
 
 int ctx_alloc(hd_ctx *ctx, void **p, size_t bytes);
 double *field_ptr(hd_ctx *ctx, int f);
+double *field_ptr(const DevState &s, int f);
 Launch2D grid2d(const DevState &s, int nscen);
 int sync_desc(hd_ctx *ctx);
 int check_launch(hd_ctx *ctx, const char *what);
@@ -70,7 +78,8 @@ int prof_resolve(hd_ctx *ctx);
         (rc) = hd::check_launch((ctx), (what));         \
         if (!(rc)) hd::prof_end((ctx), (what), e0_);    \
     } while (0)
-int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh);
+int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s);
+int alloc_shared(hd_ctx *ctx, int nscen);
 
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);

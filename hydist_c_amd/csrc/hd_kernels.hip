@@ -223,10 +223,10 @@ __global__ void k_preprocess(const DevState *__restrict__ S) {
 // a barrier between them so a later side wins at shared corner cells.
 // Only segment 0 of the boundary line receives a value (:614-618).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ S, double t_in) {
+__global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.x];
     const int M = s.M, N = s.N, W = s.P;
-    const double t = (double)(float)t_in;  // engine.cpp:22 passes t as float
+    const double t = (double)(float)s.clock[0];  // engine.cpp:22 passes t as float
     const int t1 = (int)(t / 3600);
     const double t2 = (t - (3600.0f * t1)) / 3600.0f;
     const double t2fs = (t - (3600.0 * t1)) / 3600.0;
@@ -442,6 +442,11 @@ __global__ void k_post_a(const DevState *__restrict__ S) {
     }
     // z last: other threads read z only outside [0,N]x[0,M], which is not written
     if (i <= s.N && j <= s.M) s.z[pos] = s.t_z[pos];
+}
+
+// The half-step's time: set to `v`, or advanced by `v` (graph replays).
+__global__ void k_clock(double *clock, double v, int advance) {
+    if (threadIdx.x == 0) clock[0] = advance ? clock[0] + v : v;
 }
 
 // Second half-step before Reset_states_vertical (engine.cpp:229-236):

@@ -12,16 +12,27 @@ enum { ROW_MAJOR = 0, LINE_MAJOR = 1 };
 // Device buffers of one line sweep: the assembled tridiagonal rows, the
 // solution and the Thomas scratch, plus each segment's first row and last
 // row index (sn) in the line's row space.
+// A batch of scenarios keeps one set per scenario, `stride` doubles (and
+// `sstride` segment entries) apart.
 struct SweepBufs {
     double *A, *B, *C, *D, *X, *Ap, *Bp;
     int *sys0, *sysn;
     int ld;          // leading dimension of the active layout
     int ld_rm[2];    // [dir]: lines per row in ROW_MAJOR
     int ld_lm[2];    // [dir]: rows per line in LINE_MAJOR
+    size_t stride, sstride;
+
+    __host__ __device__ SweepBufs scen(int k) const {
+        SweepBufs b = *this;
+        const size_t o = (size_t)k * stride, so = (size_t)k * sstride;
+        b.A += o; b.B += o; b.C += o; b.D += o; b.X += o; b.Ap += o; b.Bp += o;
+        b.sys0 += so; b.sysn += so;
+        return b;
+    }
 };
 
 int sweep_rows(const DevState &s, int dir, bool hyd);
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
-                   int end, double t, double s_start);
+                   int end, double s_start);
 
 }  // namespace hd

@@ -123,8 +123,9 @@ HD_DEV size_t lm_idx(int rho, int line, int ld) { return (size_t)line * ld + rho
 // layout runs against the thread order, a 64 x 16 tile is transposed
 // through LDS.
 template <int DIR, int LAYOUT>
-__global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict__ S, SweepBufs B, int start,
+__global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict__ S, SweepBufs Bin, int start,
                                                       int end) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 8 + threadIdx.y;
@@ -205,9 +206,11 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
 // FS assembly: row p-2 for cells in [f+1, l-1] of segments of >= 3 cells.
 // ---------------------------------------------------------------------------
 template <int DIR, int LAYOUT>
-__global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict__ S, SweepBufs B, int start,
-                                                     int end, double t, double s_start) {
+__global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict__ S, SweepBufs Bin, int start,
+                                                     int end, double s_start) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
+    const double t = s.clock[0];
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 8 + threadIdx.y;
     const int line = DIR == 1 ? j : i;
@@ -275,8 +278,9 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
 // ---------------------------------------------------------------------------
 constexpr int TB = 8;
 
-__global__ void __launch_bounds__(64) k_thomas(const DevState *__restrict__ S, SweepBufs B, int dir, int start,
+__global__ void __launch_bounds__(64) k_thomas(const DevState *__restrict__ S, SweepBufs Bin, int dir, int start,
                                                int end, int nrows) {
+    const SweepBufs B = Bin.scen(blockIdx.y);
     const DevState &s = S[blockIdx.y];
     const int line = start + blockIdx.x * 64 + threadIdx.x;
     const bool active = line <= end;
@@ -559,8 +563,9 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
 // recovery, its reduced z row for the residual) stay in its registers, so
 // each coefficient is read from HBM once and each unknown written once.
 template <typename T, bool REFINE, bool INTERLEAVED, int KPT>
-__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs B, int start, int end,
+__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs Bin, int start, int end,
                                            int nrows) {
+    const SweepBufs B = Bin.scen(blockIdx.y);
     extern __shared__ double smem[];
     const int line = start + blockIdx.x;
     if (line > end) return;
@@ -672,7 +677,8 @@ __global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, Swee
 // the end cell, whose continuity formula uses the new velocity at last-1.
 // ---------------------------------------------------------------------------
 template <int DIR, int LAYOUT>
-__global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs B, int start, int end) {
+__global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -726,7 +732,8 @@ __global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs B, int s
 // become NDnen; the segment's end cells take the neighbouring value or keep
 // FS as the reference's end tests decide.
 template <int DIR, int LAYOUT>
-__global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs B, int start, int end) {
+__global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -784,7 +791,7 @@ static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
 
 template <int DIR, int LAYOUT>
 static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
-                      double t, double s_start) {
+                      double s_start) {
     hipStream_t st = ctx->stream;
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
@@ -795,7 +802,7 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_hyd_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end));
     else
         LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
-               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start));
+               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, s_start));
     const int lines = end - start + 1;
     const int form = ctx->cfg.form;
     if (LAYOUT == ROW_MAJOR) {
@@ -803,7 +810,6 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
     } else {
         const int nz = hyd ? nrows / 2 : nrows;
-        if (n != 1) return HD_E_UNSUPPORTED;
         const int threads = nz > 1024 ? 512 : 256;
         const int kpt = (nz + threads - 1) / threads;
         if (kpt > 8) {
@@ -818,7 +824,7 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         auto go = [&](auto kern) {
             rc = allow_lds(ctx, kern, sm);
             if (rc) return;
-            LAUNCH_RC(ctx, rc, name, (kern<<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
         };
         auto pick = [&](auto kp) {
             constexpr int K = decltype(kp)::value;
@@ -844,14 +850,14 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
 }
 
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
-                   int end, double t, double s_start) {
+                   int end, double s_start) {
     if (end < start) return HD_OK;
     const bool plain = ctx->cfg.form == HD_FORM_FP64_PLAIN;
     if (dir == 1)
-        return plain ? sweep_impl<1, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
-                     : sweep_impl<1, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
-    return plain ? sweep_impl<0, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
-                 : sweep_impl<0, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
+        return plain ? sweep_impl<1, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, s_start)
+                     : sweep_impl<1, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, s_start);
+    return plain ? sweep_impl<0, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, s_start)
+                 : sweep_impl<0, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, s_start);
 }
 
 }  // namespace hd

@@ -51,6 +51,12 @@ def lib():
         L.hd_last_error.argtypes = [vp]
         L.hd_last_error.restype = C.c_char_p
         L.hd_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.hd_batch_create.argtypes = [C.POINTER(abi.Mesh), C.c_int, C.POINTER(abi.Coeffs), C.POINTER(abi.Options),
+                                      C.POINTER(abi.Config), C.POINTER(vp)]
+        L.hd_batch_size.argtypes = [vp, C.POINTER(C.c_int)]
+        for n in ("hd_batch_copy_out", "hd_batch_copy_in"):
+            getattr(L, n).argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
+        L.hd_set_graph.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -201,3 +207,56 @@ def stage_stats(sim, reset=True):
     buf = C.create_string_buffer(1 << 16)
     check(_stage_api().hd_stage_stats(sim.h, buf, len(buf), int(reset)), sim.h)
     return {k: (v[0], v[1]) for k, v in json.loads(buf.value.decode()).items()}
+
+
+class Batch:
+    """n scenarios of one mesh shape advanced together: every stage is one
+    launch over all of them; optionally each step is one graph launch."""
+
+    def __init__(self, scens, form=abi.HD_FORM_FP64_PLAIN, sediment=False, bed_change=False, quirks=0,
+                 Tmax=3600.0, coeffs=None, device=0, graph=False):
+        self.scens = list(scens)
+        n = len(self.scens)
+        self.shape = self.scens[0].shape
+        self.holders = [MeshHolder(s) for s in self.scens]
+        meshes = (abi.Mesh * n)(*[h.mesh for h in self.holders])
+        self.coeffs = coeffs if coeffs is not None else load_coeffs(self.scens[0].coeff_overrides)
+        self.ops = make_options(self.scens[0], Tmax)
+        self.cfg = make_config(form=form, sediment=sediment, bed_change=bed_change, quirks=quirks, device=device)
+        h = C.c_void_p()
+        check(lib().hd_batch_create(meshes, n, C.byref(self.coeffs), C.byref(self.ops), C.byref(self.cfg),
+                                    C.byref(h)))
+        self.h = h
+        self.n = n
+        self.t = 0.0
+        if graph:
+            check(lib().hd_set_graph(self.h, 1), self.h)
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            lib().hd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def init(self):
+        check(lib().hd_init(self.h), self.h)
+
+    def step(self, n=1):
+        t = C.c_double(self.t)
+        for _ in range(n):
+            check(lib().hd_step(self.h, C.byref(t)), self.h)
+        self.t = t.value
+
+    def sync(self):
+        check(lib().hd_sync(self.h), self.h)
+
+    def get(self, scen, name):
+        a = np.empty(self.shape, dtype=np.float64)
+        check(lib().hd_batch_copy_out(self.h, scen, F[name], _dp(a), a.size), self.h)
+        return a
+
+    def set(self, scen, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().hd_batch_copy_in(self.h, scen, F[name], _dp(a), a.size), self.h)

@@ -222,6 +222,31 @@ int hd_device_info(char *buf, size_t n);
 int hd_set_profiling(hd_ctx *ctx, int on);
 int hd_stage_stats(hd_ctx *ctx, char *buf, size_t n, int reset);
 
+/* ------------------------------------------------------------------ */
+/* Ensembles.  The reference runs one scenario per process (main.cpp); */
+/* these calls run n independent scenarios of one mesh shape in one    */
+/* context: every stage is one launch over all of them (the scenario   */
+/* is the grid's outer dimension), and hd_step / hd_run / hd_init /    */
+/* hd_halfstep_* advance all of them together.  Scenarios may differ   */
+/* in every field and series; M, N and total_time are shared, and so   */
+/* are coefficients, options and form.  Scenario 0 is the one the      */
+/* single-scenario calls (hd_copy_out, hd_get_segments, ...) address.  */
+/* ------------------------------------------------------------------ */
+int hd_batch_create(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, const hd_options *ops,
+                    const hd_config *cfg, hd_ctx **out);
+int hd_batch_size(hd_ctx *ctx, int *n);
+/* As hd_copy_out / hd_copy_in for scenario `scen`; HD_F_KHOUOT is
+ * returned as doubles. */
+int hd_batch_copy_out(hd_ctx *ctx, int scen, int field, double *dst, size_t n);
+int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t n);
+
+/* Whole-step graphs (engine.cpp:61-295 issues each stage as a separate
+ * launch followed by a device sync).  When on, hd_step / hd_run replay one
+ * captured hipGraph per step: one graph per combination of optional
+ * stages (sediment in either half-step, bed change), captured the first
+ * time it is needed.  Results are identical to eager launches. */
+int hd_set_graph(hd_ctx *ctx, int on);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,20 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_library_exports_the_reference_host_surface():
+    # include/hydist_compat.h: the C++ functions of engine.h / loader.h
+    import subprocess
+    out = subprocess.run(["nm", "-DC", "--defined-only", engine.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    text = open(os.path.join(ROOT, "include", "hydist_compat.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"^\s*(?:void|int|Argument_Pointers|Array_Pointers|hd_ctx \*|"
+                           r"std::pair<int, int>)\s+\*?([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, flags=re.M))
+    assert {"Hydraulic_Calculation", "load_inputs", "Load_coeffs", "save_result"} <= names
+    missing = [n for n in names if f" {n}(" not in out and f" {n}<" not in out]
+    assert not missing, missing
+
+
 def test_struct_sizes_match_reference_layout():
     # Constant_Coeffs: 49 doubles; Options: 4 doubles, int, 2 bools, 2 ints, 4 bools, int
     assert C.sizeof(abi.Coeffs) == 49 * 8
