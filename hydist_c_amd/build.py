@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhydist_amd.so")
 
-SOURCES = ["hd_kernels.hip", "hd_sweep.hip", "hd_api.hip", "hd_context.hip", "hd_batch.hip",
+SOURCES = ["hd_kernels.hip", "hd_sweep.hip", "hd_api.hip", "hd_context.hip", "hd_server.hip",
            "hd_compat.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

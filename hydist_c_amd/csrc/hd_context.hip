@@ -140,4 +140,29 @@ int alloc_shared(hd_ctx *ctx, int nscen) {
 }
 
 
+// Device bytes alloc_scenario + alloc_shared take for `nscen` scenarios of
+// this mesh (the same formulas, so admission control can use it before
+// allocating).
+size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
+    const int M = mesh->M, N = mesh->N;
+    const size_t P = dense ? M + 3 : round_up(M + 3, 32), H = N + 4;
+    const size_t ldT = round_up(N + 3, 32), capH = (M + 3) / 2 + 2, capV = (N + 3) / 2 + 2;
+    const size_t cells = P * H;
+    auto al = [](size_t b) { return b ? b : (size_t)8; };
+    size_t b = 28 * cells * 8 + cells * 4;
+    b += al(2 * (size_t)(M + N + 6) * 8) + 32 + 2 * (M + 3) * 8 + 2 * (N + 3) * 8;
+    b += 8 * al((size_t)mesh->total_time * 8);
+    b += (N + 3) * 4 + 2 * (N + 3) * capH * 4 + (M + 3) * 4 + 2 * (M + 3) * capV * 4;
+    const size_t scratch = std::max(2 * H * P, 2 * (size_t)(M + 4) * ldT);
+    b += 2 * scratch * 8 + (8 + H) * 8 + (size_t)((N >> 6) + 1) * P * 8;
+    b *= nscen;
+    const int rowsU = 2 * (N - 1), rowsV = 2 * (M - 1);
+    const size_t sw = std::max(std::max((size_t)rowsU * round_up(M + 3, 64), (size_t)rowsV * round_up(N + 3, 64)),
+                               std::max((size_t)(M + 3) * round_up(rowsU, 64), (size_t)(N + 3) * round_up(rowsV, 64)));
+    const size_t stride = (sw + 63) / 64 * 64;
+    const size_t nsys = std::max(capV * (M + 3), capH * (N + 3));
+    b += 7 * stride * nscen * 8 + 2 * nsys * nscen * 4 + 64 + sizeof(DevState) * nscen;
+    return b;
+}
+
 }  // namespace hd

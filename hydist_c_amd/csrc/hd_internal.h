@@ -80,6 +80,7 @@ int prof_resolve(hd_ctx *ctx);
     } while (0)
 int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s);
 int alloc_shared(hd_ctx *ctx, int nscen);
+size_t footprint(const hd_mesh *mesh, int dense, int nscen);
 
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);

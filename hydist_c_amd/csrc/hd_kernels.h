@@ -12,7 +12,7 @@ constexpr int SEG_COLS = 32, SEG_GROUPS = 32;
 __global__ void k_segment_cols(const DevState *S);
 __global__ void k_htuongdoi(const DevState *S);
 __global__ void k_preprocess(const DevState *S);
-__global__ void k_boundary(const DevState *S);
+__global__ void k_boundary(const DevState *S, double t, int from_clock);
 __global__ void k_clock(double *clock, double v, int advance);
 template <bool ISU> __global__ void k_smooth(const DevState *S);
 __global__ void k_solve_v(const DevState *S);

@@ -223,10 +223,12 @@ __global__ void k_preprocess(const DevState *__restrict__ S) {
 // a barrier between them so a later side wins at shared corner cells.
 // Only segment 0 of the boundary line receives a value (:614-618).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ S) {
+// t_in is the half-step's time, or (from_clock, graph replays) the device
+// clock holds it.
+__global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ S, double t_in, int from_clock) {
     const DevState &s = S[blockIdx.x];
     const int M = s.M, N = s.N, W = s.P;
-    const double t = (double)(float)s.clock[0];  // engine.cpp:22 passes t as float
+    const double t = (double)(float)(from_clock ? s.clock[0] : t_in);  // engine.cpp:22 passes t as float
     const int t1 = (int)(t / 3600);
     const double t2 = (t - (3600.0f * t1)) / 3600.0f;
     const double t2fs = (t - (3600.0 * t1)) / 3600.0;

@@ -33,6 +33,6 @@ struct SweepBufs {
 
 int sweep_rows(const DevState &s, int dir, bool hyd);
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
-                   int end, double s_start);
+                   int end, double t, double s_start);
 
 }  // namespace hd

@@ -207,10 +207,10 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
 // ---------------------------------------------------------------------------
 template <int DIR, int LAYOUT>
 __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict__ S, SweepBufs Bin, int start,
-                                                     int end, double s_start) {
+                                                     int end, double t_in, double s_start, int from_clock) {
     const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
-    const double t = s.clock[0];
+    const double t = from_clock ? s.clock[0] : t_in;
     const int j = blockIdx.x * 64 + threadIdx.x;
     const int i = blockIdx.y * 8 + threadIdx.y;
     const int line = DIR == 1 ? j : i;
@@ -558,15 +558,93 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
     return o;
 }
 
+// The FP64 reduced rows (for the residual) are kept in the Ap/Bp scratch.
+template <typename T, bool REFINE, bool INTERLEAVED>
+__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs B, int start, int end,
+                                           int nrows) {
+    extern __shared__ unsigned char smem[];
+    const int line = start + blockIdx.x;
+    if (line > end) return;
+    const int ld = B.ld;
+    const double *__restrict__ A = B.A + (size_t)line * ld;
+    const double *__restrict__ Bv = B.B + (size_t)line * ld;
+    const double *__restrict__ C = B.C + (size_t)line * ld;
+    const double *__restrict__ D = B.D + (size_t)line * ld;
+    double *__restrict__ X = B.X + (size_t)line * ld;
+    // FP64 reduced system kept for the residual (global scratch Ap/Bp: a,b | c,d)
+    double *__restrict__ Ra = B.Ap + (size_t)line * ld;
+    double *__restrict__ Rb = Ra + ld / 2;
+    double *__restrict__ Rc = B.Bp + (size_t)line * ld;
+    double *__restrict__ Rd = Rc + ld / 2;
+    const int n = INTERLEAVED ? nrows / 2 : nrows;
+    T *base = reinterpret_cast<T *>(smem);
+    CRShared<T> sh{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n};
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int q = tid; q < n; q += nt) {
+        Row r;
+        if (INTERLEAVED)
+            r = reduce_z_row(A, Bv, C, D, q, nrows);
+        else
+            r = Row{A[q], Bv[q], C[q], D[q]};
+        sh.a[q] = (T)r.a;
+        sh.b[q] = (T)r.b;
+        sh.c[q] = (T)r.c;
+        sh.d[q] = (T)r.d;
+        if (REFINE) {
+            Ra[q] = r.a;
+            Rb[q] = r.b;
+            Rc[q] = r.c;
+            Rd[q] = r.d;
+        }
+    }
+    __syncthreads();
+    cr_solve<T>(sh, n, REFINE);
+    if (REFINE) {
+        // z0 -> X (FP64), residual of the FP64 system -> d
+        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+        __syncthreads();
+        for (int q = tid; q < n; q += nt) {
+            const int stp = INTERLEAVED ? 2 : 1;
+            const double xq = X[stp * q];
+            double rr = Rd[q] - Rb[q] * xq;
+            if (q > 0) rr = rr - Ra[q] * X[stp * (q - 1)];
+            if (q + 1 < n) rr = rr - Rc[q] * X[stp * (q + 1)];
+            sh.d[q] = (T)rr;
+        }
+        __syncthreads();
+        cr_replay<T>(sh, n);
+        for (int q = tid; q < n; q += nt) {
+            const int stp = INTERLEAVED ? 2 : 1;
+            X[stp * q] = X[stp * q] + (double)sh.d[q];
+        }
+    } else {
+        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
+    }
+    if (INTERLEAVED) {
+        __syncthreads();
+        // velocity rows from their own FP64 rows
+        for (int q = tid; q < n; q += nt) {
+            const int r = 2 * q + 1;
+            if (r >= nrows) continue;
+            double v = D[r];
+            const double a = A[r], c = C[r];
+            if (a != 0.0) v = v - a * X[r - 1];
+            if (c != 0.0 && r + 1 < nrows) v = v - c * X[r + 1];
+            X[r] = v / Bv[r];
+        }
+    }
+}
+
+// Register-resident variant (not enabled: see kRegisterCR).
 // One workgroup per line; thread t owns rows q = t + k * blockDim.x,
 // k < KPT.  The FP64 rows a thread needs again (its velocity row for the
 // recovery, its reduced z row for the residual) stay in its registers, so
 // each coefficient is read from HBM once and each unknown written once.
 template <typename T, bool REFINE, bool INTERLEAVED, int KPT>
-__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs Bin, int start, int end,
+__global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, SweepBufs Bin, int start, int end,
                                            int nrows) {
     const SweepBufs B = Bin.scen(blockIdx.y);
-    extern __shared__ double smem[];
+    extern __shared__ double smem_reg[];
     const int line = start + blockIdx.x;
     if (line > end) return;
     const int ld = B.ld;
@@ -576,7 +654,7 @@ __global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, Swee
     const double *__restrict__ D = B.D + (size_t)line * ld;
     double *__restrict__ X = B.X + (size_t)line * ld;
     const int n = INTERLEAVED ? nrows / 2 : nrows;
-    T *base = reinterpret_cast<T *>(smem);
+    T *base = reinterpret_cast<T *>(smem_reg);
     CRShared<T> sh{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n};
     const int tid = threadIdx.x, nt = blockDim.x;
     Row red[KPT], vel[KPT];
@@ -648,7 +726,7 @@ __global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, Swee
     // double sh.d already holds it) for the neighbours' velocity recovery
     const double *zs = reinterpret_cast<const double *>(sh.d);
     if (REFINE) {
-        double *zw = smem;
+        double *zw = smem_reg;
 #pragma unroll
         for (int k = 0; k < KPT; k++) {
             const int q = tid + k * nt;
@@ -778,6 +856,11 @@ int sweep_rows(const DevState &s, int dir, bool hyd) {
 
 // Lets a kernel use more than the default 64 KiB of dynamic LDS (gfx950
 // has 160 KiB per CU).
+// The register-resident solver (k_cr_reg) is switched off: the first GPU
+// run that contained it (with batched contexts and whole-step graphs) ended
+// in a device fault whose cause has not been found.
+constexpr bool kRegisterCR = false;
+
 template <typename K>
 static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
     if (bytes <= 65536) return HD_OK;
@@ -791,7 +874,7 @@ static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
 
 template <int DIR, int LAYOUT>
 static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
-                      double s_start) {
+                      double t, double s_start) {
     hipStream_t st = ctx->stream;
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
@@ -802,42 +885,70 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_hyd_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end));
     else
         LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
-               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, s_start));
+               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, ctx->capturing ? 1 : 0));
     const int lines = end - start + 1;
     const int form = ctx->cfg.form;
     if (LAYOUT == ROW_MAJOR) {
         LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
     } else {
-        const int nz = hyd ? nrows / 2 : nrows;
-        const int threads = nz > 1024 ? 512 : 256;
-        const int kpt = (nz + threads - 1) / threads;
-        if (kpt > 8) {
-            ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the solver's 4096";
-            return HD_E_UNSUPPORTED;
-        }
-        const bool f32 = form == HD_FORM_FP32_REFINE;
-        const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
-        const char *name = f32 ? (hyd ? (DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>") : (DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>"))
-                               : (hyd ? (DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>") : (DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>"));
-        int rc = HD_OK;
-        auto go = [&](auto kern) {
-            rc = allow_lds(ctx, kern, sm);
-            if (rc) return;
-            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-        };
-        auto pick = [&](auto kp) {
-            constexpr int K = decltype(kp)::value;
-            if (f32) {
-                if (hyd) go(k_cr<float, true, true, K>); else go(k_cr<float, true, false, K>);
+        if (!kRegisterCR) {
+            const int nz = hyd ? nrows / 2 : nrows;
+            const int threads = nz > 1024 ? 512 : 256;
+            if (form == HD_FORM_FP32_REFINE) {
+                const size_t sm = 6 * (size_t)nz * sizeof(float);
+                if (n != 1) return HD_E_UNSUPPORTED;
+                int rc = allow_lds(ctx, hyd ? k_cr<float, true, true> : k_cr<float, true, false>, sm);
+                if (rc) return rc;
+                if (hyd)
+                    LAUNCH(ctx, DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>",
+                           (k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                else
+                    LAUNCH(ctx, DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>",
+                           (k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
             } else {
-                if (hyd) go(k_cr<double, false, true, K>); else go(k_cr<double, false, false, K>);
+                const size_t sm = 4 * (size_t)nz * sizeof(double);
+                if (n != 1) return HD_E_UNSUPPORTED;
+                int rc = allow_lds(ctx, hyd ? k_cr<double, false, true> : k_cr<double, false, false>, sm);
+                if (rc) return rc;
+                if (hyd)
+                    LAUNCH(ctx, DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>",
+                           (k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                else
+                    LAUNCH(ctx, DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>",
+                           (k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
             }
-        };
-        if (kpt <= 2) pick(std::integral_constant<int, 2>{});
-        else if (kpt <= 4) pick(std::integral_constant<int, 4>{});
-        else pick(std::integral_constant<int, 8>{});
-        if (rc) return rc;
+        } else {
+            const int nz = hyd ? nrows / 2 : nrows;
+            const int threads = nz > 1024 ? 512 : 256;
+            const int kpt = (nz + threads - 1) / threads;
+            if (kpt > 8) {
+                ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the solver's 4096";
+                return HD_E_UNSUPPORTED;
+            }
+            const bool f32 = form == HD_FORM_FP32_REFINE;
+            const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
+            const char *name = f32 ? (hyd ? (DIR ? "k_cr_reg<f32r,uz>" : "k_cr_reg<f32r,vz>") : (DIR ? "k_cr_reg<f32r,fs_j>" : "k_cr_reg<f32r,fs_i>"))
+                                   : (hyd ? (DIR ? "k_cr_reg<f64,uz>" : "k_cr_reg<f64,vz>") : (DIR ? "k_cr_reg<f64,fs_j>" : "k_cr_reg<f64,fs_i>"));
+            int rc = HD_OK;
+            auto go = [&](auto kern) {
+                rc = allow_lds(ctx, kern, sm);
+                if (rc) return;
+                LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
+            };
+            auto pick = [&](auto kp) {
+                constexpr int K = decltype(kp)::value;
+                if (f32) {
+                    if (hyd) go(k_cr_reg<float, true, true, K>); else go(k_cr_reg<float, true, false, K>);
+                } else {
+                    if (hyd) go(k_cr_reg<double, false, true, K>); else go(k_cr_reg<double, false, false, K>);
+                }
+            };
+            if (kpt <= 2) pick(std::integral_constant<int, 2>{});
+            else if (kpt <= 4) pick(std::integral_constant<int, 4>{});
+            else pick(std::integral_constant<int, 8>{});
+            if (rc) return rc;
+        }
     }
     const Launch2D g = grid2d(s, n);
     if (hyd)
@@ -850,14 +961,14 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
 }
 
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
-                   int end, double s_start) {
+                   int end, double t, double s_start) {
     if (end < start) return HD_OK;
     const bool plain = ctx->cfg.form == HD_FORM_FP64_PLAIN;
     if (dir == 1)
-        return plain ? sweep_impl<1, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, s_start)
-                     : sweep_impl<1, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, s_start);
-    return plain ? sweep_impl<0, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, s_start)
-                 : sweep_impl<0, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, s_start);
+        return plain ? sweep_impl<1, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
+                     : sweep_impl<1, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
+    return plain ? sweep_impl<0, ROW_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start)
+                 : sweep_impl<0, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
 }
 
 }  // namespace hd

@@ -260,3 +260,109 @@ class Batch:
     def set(self, scen, name, a):
         a = np.ascontiguousarray(a, dtype=np.float64)
         check(lib().hd_batch_copy_in(self.h, scen, F[name], _dp(a), a.size), self.h)
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("steps_per_quantum", C.c_int),
+                ("use_graph", C.c_int), ("device", C.c_int)]
+
+
+RUN_QUEUED, RUN_RUNNING, RUN_DONE, RUN_FAILED, RUN_CANCELLED = range(5)
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_bound", False):
+        vp, i64 = C.c_void_p, C.c_int64
+        L.hd_server_create.argtypes = [C.POINTER(_ServerConfig), C.POINTER(vp)]
+        L.hd_server_submit.argtypes = [vp, C.POINTER(abi.Mesh), C.POINTER(abi.Coeffs), C.POINTER(abi.Options),
+                                       C.POINTER(abi.Config), C.POINTER(i64)]
+        L.hd_server_poll.argtypes = [vp, i64, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.hd_server_wait.argtypes = [vp, i64, C.c_double, C.POINTER(C.c_int)]
+        L.hd_server_result.argtypes = [vp, i64, C.c_int, C.POINTER(C.c_double), C.c_size_t]
+        L.hd_server_error.argtypes = [vp, i64, C.c_char_p, C.c_size_t]
+        for n in ("hd_server_cancel", "hd_server_release"):
+            getattr(L, n).argtypes = [vp, i64]
+        L.hd_server_stats.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.hd_server_destroy.argtypes = [vp]
+        L.hd_footprint.argtypes = [C.POINTER(abi.Mesh), C.POINTER(abi.Config), C.c_int, C.POINTER(C.c_size_t)]
+        L.hd_memory_bytes.argtypes = [vp, C.POINTER(C.c_size_t)]
+        L._server_bound = True
+    return L
+
+
+def footprint(scen, nscen=1, dense_layout=0):
+    L = _server_lib()
+    holder = MeshHolder(scen)
+    cfg = make_config()
+    cfg.dense_layout = dense_layout
+    out = C.c_size_t()
+    check(L.hd_footprint(C.byref(holder.mesh), C.byref(cfg), nscen, C.byref(out)))
+    return out.value
+
+
+class Server:
+    """The serving loop: submit what-if runs as they arrive; they are
+    admitted by HBM footprint and advanced together on the GPU."""
+
+    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_quantum=16, use_graph=True, device=0):
+        L = _server_lib()
+        cfg = _ServerConfig(int(hbm_budget), max_active, steps_per_quantum, int(use_graph), device)
+        h = C.c_void_p()
+        check(L.hd_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.shapes = {}
+
+    def submit(self, scen, Tmax, form=abi.HD_FORM_FP64_PLAIN, sediment=False, bed_change=False, coeffs=None):
+        L = _server_lib()
+        holder = MeshHolder(scen)
+        c = coeffs if coeffs is not None else load_coeffs(scen.coeff_overrides)
+        ops = make_options(scen, Tmax)
+        cfg = make_config(form=form, sediment=sediment, bed_change=bed_change)
+        rid = C.c_int64()
+        check(L.hd_server_submit(self.h, C.byref(holder.mesh), C.byref(c), C.byref(ops), C.byref(cfg),
+                                 C.byref(rid)))
+        self.shapes[rid.value] = scen.shape
+        return rid.value
+
+    def poll(self, rid):
+        st, t = C.c_int(), C.c_double()
+        check(_server_lib().hd_server_poll(self.h, rid, C.byref(st), C.byref(t)))
+        return st.value, t.value
+
+    def wait(self, rid, timeout=-1.0):
+        st = C.c_int()
+        _server_lib().hd_server_wait(self.h, rid, timeout, C.byref(st))
+        return st.value
+
+    def result(self, rid, name):
+        a = np.empty(self.shapes[rid], dtype=np.float64)
+        rc = _server_lib().hd_server_result(self.h, rid, F[name], _dp(a), a.size)
+        if rc != 0:
+            raise HydistError(f"run {rid}: result unavailable ({rc}): {self.error(rid)}")
+        return a
+
+    def error(self, rid):
+        buf = C.create_string_buffer(512)
+        _server_lib().hd_server_error(self.h, rid, buf, 512)
+        return buf.value.decode(errors="replace")
+
+    def cancel(self, rid):
+        check(_server_lib().hd_server_cancel(self.h, rid))
+
+    def release(self, rid):
+        check(_server_lib().hd_server_release(self.h, rid))
+
+    def stats(self):
+        import json
+        buf = C.create_string_buffer(2048)
+        check(_server_lib().hd_server_stats(self.h, buf, 2048))
+        return json.loads(buf.value.decode())
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            _server_lib().hd_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

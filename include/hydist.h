@@ -247,6 +247,52 @@ int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t
  * time it is needed.  Results are identical to eager launches. */
 int hd_set_graph(hd_ctx *ctx, int on);
 
+/* Device bytes a context of nscen scenarios of this mesh allocates
+ * (hd_create / hd_batch_create), and what a context holds.  The
+ * reference allocates without accounting (loader.cpp:224-375). */
+int hd_footprint(const hd_mesh *mesh, const hd_config *cfg, int nscen, size_t *bytes);
+int hd_memory_bytes(hd_ctx *ctx, size_t *bytes);
+
+/* ------------------------------------------------------------------ */
+/* Serving loop.  No counterpart in the reference (one scenario per    */
+/* process, main.cpp): what-if runs are submitted as they arrive; a    */
+/* worker thread admits them in order while their footprint fits the   */
+/* HBM budget and a slot is free, keeps each admitted run resident in  */
+/* its own context and stream, and advances all admitted runs together */
+/* in rounds of steps_per_quantum steps (their kernels overlap on the  */
+/* device).  A run's result does not depend on what else runs.         */
+/* ------------------------------------------------------------------ */
+typedef struct hd_server hd_server;
+
+typedef struct hd_server_config {
+    size_t hbm_budget;      /* device bytes the admitted runs may hold together */
+    int max_active;         /* admitted runs at most */
+    int steps_per_quantum;  /* steps per run per round */
+    int use_graph;          /* 1: each step is one graph launch */
+    int device;
+} hd_server_config;
+
+enum { HD_RUN_QUEUED = 0, HD_RUN_RUNNING = 1, HD_RUN_DONE = 2, HD_RUN_FAILED = 3, HD_RUN_CANCELLED = 4 };
+
+int hd_server_create(const hd_server_config *cfg, hd_server **out);
+/* Copies the inputs; the run goes from ops->t_start until t >= (int)ops->Tmax. */
+int hd_server_submit(hd_server *s, const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops,
+                     const hd_config *cfg, int64_t *id);
+int hd_server_poll(hd_server *s, int64_t id, int *state, double *t);
+/* Waits until the run ends (timeout_s < 0: no limit); HD_E_STATE on timeout. */
+int hd_server_wait(hd_server *s, int64_t id, double timeout_s, int *state);
+/* Final U, V, Z, FS, DH or KHOUOT (as doubles) of a finished run, dense
+ * (N+3) x (M+3). */
+int hd_server_result(hd_server *s, int64_t id, int field, double *dst, size_t n);
+int hd_server_error(hd_server *s, int64_t id, char *buf, size_t n);
+int hd_server_cancel(hd_server *s, int64_t id);
+/* Drops a finished run's stored results. */
+int hd_server_release(hd_server *s, int64_t id);
+/* JSON: counts, p50/p99/mean completion latency (s, submit to results on
+ * the host), cell-steps of completed runs and their rate. */
+int hd_server_stats(hd_server *s, char *buf, size_t n);
+int hd_server_destroy(hd_server *s);
+
 #ifdef __cplusplus
 }
 #endif

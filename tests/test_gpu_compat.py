@@ -17,7 +17,7 @@ from parity_util import HYDRO, diff_report, make_pair
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "tests", "_build", "compat_driver")
+DRIVER = os.path.join(ROOT, "tests", "_build", "emu" if os.environ.get("HD_EMU") == "1" else "", "compat_driver")
 
 
 def read_grid(path):
