@@ -779,6 +779,140 @@ int hd_memory_bytes(hd_ctx *ctx, size_t *bytes) {
     return HD_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Checkpoint / restart
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SnapHeader {
+    char magic[8];  // "HDSNAP1"
+    int32_t M, N, P, H, capH, capV, nscen, nbuf;
+    double t;
+    uint64_t bytes;
+};
+
+// Every buffer whose contents carry over from one step to the next.
+std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
+    std::vector<std::pair<void *, size_t>> b;
+    for (int k = 0; k < ctx->nscen; k++) {
+        DevState &s = k == 0 ? ctx->hs : ctx->hss[k - 1];
+        const size_t cells = (size_t)s.P * s.H, fb = cells * sizeof(double);
+        for (double *f : {s.u, s.v, s.z, s.t_u, s.t_v, s.t_z, s.H_moi, s.Htdu, s.Htdv, s.FS, s.tFS, s.VTH, s.Kx, s.Ky,
+                          s.Qbx, s.Qby, s.dH, s.htaiz, s.htaiz_bd, s.Kx1, s.Ky1})
+            b.push_back({f, fb});
+        b.push_back({s.khouot, cells * sizeof(int)});
+        b.push_back({s.hi, 2 * (size_t)(s.M + s.N + 6) * sizeof(double)});
+        b.push_back({s.hmax, 4 * sizeof(double)});
+        b.push_back({s.ubt, (s.M + 3) * sizeof(double)});
+        b.push_back({s.ubp, (s.M + 3) * sizeof(double)});
+        b.push_back({s.vbt, (s.N + 3) * sizeof(double)});
+        b.push_back({s.vbd, (s.N + 3) * sizeof(double)});
+        b.push_back({s.segH_n, (s.N + 3) * sizeof(int)});
+        b.push_back({s.segH_f, (size_t)(s.N + 3) * s.capH * sizeof(int)});
+        b.push_back({s.segH_l, (size_t)(s.N + 3) * s.capH * sizeof(int)});
+        b.push_back({s.segV_n, (s.M + 3) * sizeof(int)});
+        b.push_back({s.segV_f, (size_t)(s.M + 3) * s.capV * sizeof(int)});
+        b.push_back({s.segV_l, (size_t)(s.M + 3) * s.capV * sizeof(int)});
+        b.push_back({s.acc, 8 * sizeof(double)});
+    }
+    return b;
+}
+
+SnapHeader snap_header(hd_ctx *ctx, double t, size_t nbuf, size_t bytes) {
+    SnapHeader h{};
+    std::memcpy(h.magic, "HDSNAP1", 8);
+    const DevState &s = ctx->hs;
+    h.M = s.M; h.N = s.N; h.P = s.P; h.H = s.H; h.capH = s.capH; h.capV = s.capV;
+    h.nscen = ctx->nscen;
+    h.nbuf = (int32_t)nbuf;
+    h.t = t;
+    h.bytes = bytes;
+    return h;
+}
+
+}  // namespace
+
+int hd_snapshot_size(hd_ctx *ctx, size_t *bytes) {
+    if (!ctx || !bytes) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    size_t n = sizeof(SnapHeader);
+    for (auto &b : snap_buffers(ctx)) n += b.second;
+    *bytes = n;
+    return HD_OK;
+}
+
+int hd_snapshot(hd_ctx *ctx, void *buf, size_t n, double t) {
+    if (!ctx || !buf) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    size_t need = 0;
+    int rc = hd_snapshot_size(ctx, &need);
+    if (rc) return rc;
+    if (n < need) return set_err(ctx, HD_E_INVALID, "snapshot buffer too small");
+    auto bufs = snap_buffers(ctx);
+    SnapHeader h = snap_header(ctx, t, bufs.size(), need);
+    char *p = static_cast<char *>(buf);
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &b : bufs) {
+        HD_HIP(ctx, hipMemcpy(p, b.first, b.second, hipMemcpyDeviceToHost));
+        p += b.second;
+    }
+    return HD_OK;
+}
+
+int hd_restore(hd_ctx *ctx, const void *buf, size_t n, double *t) {
+    if (!ctx || !buf) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    if (n < sizeof(SnapHeader)) return set_err(ctx, HD_E_INVALID, "snapshot too short");
+    SnapHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    auto bufs = snap_buffers(ctx);
+    size_t need = 0;
+    int rc = hd_snapshot_size(ctx, &need);
+    if (rc) return rc;
+    const SnapHeader want = snap_header(ctx, h.t, bufs.size(), need);
+    if (std::memcmp(h.magic, want.magic, 8) || h.M != want.M || h.N != want.N || h.P != want.P || h.H != want.H ||
+        h.capH != want.capH || h.capV != want.capV || h.nscen != want.nscen || h.nbuf != want.nbuf ||
+        h.bytes != need || n < need)
+        return set_err(ctx, HD_E_INVALID, "snapshot does not match this context's mesh and layout");
+    const char *p = static_cast<const char *>(buf) + sizeof(h);
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto &b : bufs) {
+        HD_HIP(ctx, hipMemcpy(b.first, p, b.second, hipMemcpyHostToDevice));
+        p += b.second;
+    }
+    ctx->inited = true;
+    if (t) *t = h.t;
+    return HD_OK;
+}
+
+int hd_save_checkpoint(hd_ctx *ctx, const char *path, double t) {
+    if (!ctx || !path) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    size_t n = 0;
+    int rc = hd_snapshot_size(ctx, &n);
+    if (rc) return rc;
+    std::vector<char> buf(n);
+    rc = hd_snapshot(ctx, buf.data(), n, t);
+    if (rc) return rc;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return set_err(ctx, HD_E_INVALID, std::string("cannot open ") + path);
+    const size_t w = std::fwrite(buf.data(), 1, n, f);
+    const int c = std::fclose(f);
+    if (w != n || c != 0) return set_err(ctx, HD_E_INVALID, std::string("write failed: ") + path);
+    return HD_OK;
+}
+
+int hd_load_checkpoint(hd_ctx *ctx, const char *path, double *t) {
+    if (!ctx || !path) return set_err(ctx, HD_E_INVALID, "NULL argument");
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return set_err(ctx, HD_E_INVALID, std::string("cannot open ") + path);
+    std::vector<char> buf;
+    char chunk[1 << 16];
+    size_t r;
+    while ((r = std::fread(chunk, 1, sizeof(chunk), f)) > 0) buf.insert(buf.end(), chunk, chunk + r);
+    std::fclose(f);
+    return hd_restore(ctx, buf.data(), buf.size(), t);
+}
+
 int hd_set_graph(hd_ctx *ctx, int on) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (on && !kGraphEnabled) return set_err(ctx, HD_E_UNSUPPORTED, "whole-step graphs are switched off in this build");

@@ -568,10 +568,13 @@ __global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__
         for (int j = 1 + threadIdx.x; j < M; j += blockDim.x) {
             if (s.t_u[location * W + j] > 0) {
                 double deltaX = (s.u[location * W + j] + s.t_u[location * W + j]) * 0.25 * dT;
-                int i = (int)(offset + sign * (deltaX / dX));
+                // source rows outside the field, or not finite, are skipped;
+                // tested before any conversion to int
+                const double x = offset + sign * (deltaX / dX);
+                if (!(x > -1.0 && x < N + 2.0)) continue;
+                int i = (int)x;
                 int i_deltaX = __double2int_rn(deltaX);
                 int i_dX = __double2int_rn(dX);
-                if (i < 0 || i + 1 > N + 2) continue;
                 FS[location * W + j] =
                     FS[i * W + j] + sign * (FS[(i + 1) * W + j] - FS[i * W + j]) * (deltaX / dX - i_deltaX / i_dX);
             }
@@ -583,10 +586,11 @@ __global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__
     for (int i = 1 + threadIdx.x; i <= iend; i += blockDim.x) {
         if (s.t_v[i * W + location] > 0) {
             double deltaY = (s.v[i * W + location] + s.t_v[i * W + location]) * 0.25 * dT;
-            int j = (int)(offset + sign * (deltaY / dY));
+            const double y = offset + sign * (deltaY / dY);
+            if (!(y > -1.0 && y < M + 2.0)) continue;
+            int j = (int)y;
             int j_deltaY = __double2int_rn(deltaY);
             int j_dY = __double2int_rn(dY);
-            if (j < 0 || j + 1 > M + 2) continue;
             FS[i * W + location] = FS[i * W + j + 1] + (FS[i * W + j + 1] - FS[i * W + j]) * (deltaY / dY - j_deltaY / j_dY);
         }
     }

@@ -57,6 +57,11 @@ def lib():
         for n in ("hd_batch_copy_out", "hd_batch_copy_in"):
             getattr(L, n).argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
         L.hd_set_graph.argtypes = [vp, C.c_int]
+        L.hd_snapshot_size.argtypes = [vp, C.POINTER(C.c_size_t)]
+        L.hd_snapshot.argtypes = [vp, C.c_void_p, C.c_size_t, C.c_double]
+        L.hd_restore.argtypes = [vp, C.c_void_p, C.c_size_t, dp]
+        L.hd_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_double]
+        L.hd_load_checkpoint.argtypes = [vp, C.c_char_p, dp]
         _lib = L
     return _lib
 
@@ -182,6 +187,15 @@ class Simulation:
         check(lib().hd_get_segments(self.h, direction, ipc(counts), nlines, ipc(first), ipc(last), n,
                                     C.byref(total)), self.h)
         return counts, first[:n], last[:n]
+
+    def save_checkpoint(self, path):
+        """Writes the full state and time (restart: continue bit for bit)."""
+        check(lib().hd_save_checkpoint(self.h, str(path).encode(), self.t), self.h)
+
+    def load_checkpoint(self, path):
+        t = C.c_double()
+        check(lib().hd_load_checkpoint(self.h, str(path).encode(), C.byref(t)), self.h)
+        self.t = t.value
 
     def mass_balance(self):
         v, q = C.c_double(), C.c_double()

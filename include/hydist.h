@@ -247,6 +247,19 @@ int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t
  * time it is needed.  Results are identical to eager launches. */
 int hd_set_graph(hd_ctx *ctx, int on);
 
+/* Checkpoint / restart.  Replaces the reference's stubs (the -c/-T
+ * options, main.cpp:283-284, loader.cpp:172-185): the snapshot holds every
+ * buffer whose contents carry from one step to the next (state, derived
+ * fields, boundary arrays, segment tables, inflow accumulator) and the
+ * time t, in the device layout; it restores into a context created from
+ * the same mesh and config, and the run then continues bit for bit as if
+ * never interrupted. */
+int hd_snapshot_size(hd_ctx *ctx, size_t *bytes);
+int hd_snapshot(hd_ctx *ctx, void *buf, size_t n, double t);
+int hd_restore(hd_ctx *ctx, const void *buf, size_t n, double *t);
+int hd_save_checkpoint(hd_ctx *ctx, const char *path, double t);
+int hd_load_checkpoint(hd_ctx *ctx, const char *path, double *t);
+
 /* Device bytes a context of nscen scenarios of this mesh allocates
  * (hd_create / hd_batch_create), and what a context holds.  The
  * reference allocates without accounting (loader.cpp:224-375). */

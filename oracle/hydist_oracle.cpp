@@ -189,10 +189,14 @@ void interpolate_fs_ng(Oracle &o, int location, int offset, int sign) {
     for (int j = 1; j < M; j++) {
         if (o.t_u[location * W + j] > 0) {
             double deltaX = (o.u[location * W + j] + o.t_u[location * W + j]) * 0.25 * dT;
-            int i = (int)(offset + sign * (deltaX / dX));
+            // Define: a source row outside the field (i < 0 or i + 1 > N + 2
+            // after truncation), or not finite, is skipped; tested on the
+            // double so no out-of-range value is converted to int
+            const double x = offset + sign * (deltaX / dX);
+            if (!(x > -1.0 && x < o.N + 2.0)) continue;
+            int i = (int)x;
             int i_deltaX = (int)std::nearbyint(deltaX);
             int i_dX = (int)std::nearbyint(dX);
-            if (i < 0 || i + 1 > o.N + 2) continue;  // Define: rows outside the field are skipped
             o.FS[location * W + j] =
                 o.FS[i * W + j] +
                 sign * (o.FS[(i + 1) * W + j] - o.FS[i * W + j]) * (deltaX / dX - i_deltaX / i_dX);
@@ -209,10 +213,11 @@ void interpolate_fs_d(Oracle &o, int location, int offset, int sign) {
     for (int i = 1; i <= iend; i++) {
         if (o.t_v[i * W + location] > 0) {
             double deltaY = (o.v[i * W + location] + o.t_v[i * W + location]) * 0.25 * dT;
-            int j = (int)(offset + sign * (deltaY / dY));
+            const double y = offset + sign * (deltaY / dY);
+            if (!(y > -1.0 && y < o.M + 2.0)) continue;  // Define: as in interpolate_fs_ng
+            int j = (int)y;
             int j_deltaY = (int)std::nearbyint(deltaY);
             int j_dY = (int)std::nearbyint(dY);
-            if (j < 0 || j + 1 > o.M + 2) continue;
             o.FS[i * W + location] =
                 o.FS[i * W + j + 1] +
                 (o.FS[i * W + j + 1] - o.FS[i * W + j]) * (deltaY / dY - j_deltaY / j_dY);
