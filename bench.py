@@ -107,6 +107,8 @@ def time_form(sc, form, steps, warmup, profile_steps):
     # advance to the sediment switch-on, then the warm-up steps
     while sim.t < sc.sediment_start:
         sim.step()
+    # solver workgroup sizes (results do not depend on them)
+    tune = engine.autotune(sim) if form != 0 else None
     sim.step(warmup)
     sim.sync()
     t0 = time.perf_counter()
@@ -122,7 +124,7 @@ def time_form(sc, form, steps, warmup, profile_steps):
     engine.set_profiling(sim, False)
     z = sim.get("Z")
     finite = bool((z == z).all())
-    return sim, ms_per_step, stats, finite
+    return sim, ms_per_step, stats, finite, tune
 
 
 def cpu_baseline(sc, seconds):
@@ -182,9 +184,10 @@ def main():
     for name in names:
         log(f"[bench] form {name}: warm-up to t={sc.sediment_start:.0f}s + {args.warmup} steps, "
             f"then {args.steps} timed steps")
-        sim, ms, stats, finite = time_form(sc, forms[name], args.steps, args.warmup, args.profile_steps)
+        sim, ms, stats, finite, tune = time_form(sc, forms[name], args.steps, args.warmup, args.profile_steps)
         cs = sc.M * sc.N / (ms * 1e-3)
-        results[name] = {"cell_steps_per_s": cs, "ms_per_step": ms, "finite": finite, "stats": stats, "sim": sim}
+        results[name] = {"cell_steps_per_s": cs, "ms_per_step": ms, "finite": finite, "stats": stats, "sim": sim,
+                         "tune": tune}
         log(f"[bench] form {name}: {ms:.3f} ms/step, {cs:.3e} cell-steps/s")
         if best is None or cs > results[best]["cell_steps_per_s"]:
             best = name
@@ -245,7 +248,7 @@ def main():
         "config": {"workload": "2048x1024 meander+islands, sediment + FP64 bed change, dt=2s",
                    "M": sc.M, "N": sc.N, "form": best, "timed_from_t": sc.sediment_start},
         "forms": {k: {"cell_steps_per_s": v["cell_steps_per_s"], "ms_per_step": v["ms_per_step"],
-                      "finite": v["finite"],
+                      "finite": v["finite"], "solver_threads": v["tune"]["chosen"] if v["tune"] else None,
                       "profile_top": sorted(({"stage": a, "launches": b[0], "avg_ms": b[1] / b[0]}
                                              for a, b in v["stats"].items()), key=lambda d: -d["avg_ms"])[:6]}
                   for k, v in results.items()},

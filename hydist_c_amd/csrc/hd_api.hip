@@ -913,6 +913,27 @@ int hd_load_checkpoint(hd_ctx *ctx, const char *path, double *t) {
     return hd_restore(ctx, buf.data(), buf.size(), t);
 }
 
+int hd_autotune(hd_ctx *ctx, char *report, size_t n) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    std::string r = "{";
+    int rc = hd::tune_line_solvers(ctx, ctx->ds, ctx->hs, ctx->nscen, r);
+    if (rc) return rc;
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "%s\"chosen\": [%d, %d, %d, %d]}", r.size() > 1 ? ", " : "",
+                  ctx->cr_threads[1][1], ctx->cr_threads[0][1], ctx->cr_threads[1][0], ctx->cr_threads[0][0]);
+    r += buf;
+    if (report && n) std::snprintf(report, n, "%s", r.c_str());
+    return HD_OK;
+}
+
+int hd_set_solver_threads(hd_ctx *ctx, int dir, int hydraulic, int threads) {
+    if (!ctx || (dir != 0 && dir != 1) || threads < 0 || threads > 512 || threads % 64)
+        return set_err(ctx, HD_E_INVALID, "dir must be 0/1 and threads a multiple of 64 up to 512 (0 = default)");
+    ctx->cr_threads[dir][hydraulic ? 1 : 0] = threads;
+    return HD_OK;
+}
+
 int hd_set_graph(hd_ctx *ctx, int on) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (on && !kGraphEnabled) return set_err(ctx, HD_E_UNSUPPORTED, "whole-step graphs are switched off in this build");

@@ -24,6 +24,9 @@ struct hd_ctx {
     hd::DevState *ds = nullptr;    // descriptors of all scenarios (device copy)
     double *clock = nullptr;       // device time of the current half-step
     // whole-step graphs, keyed by which optional stages the step runs
+    // cyclic-reduction workgroup size per sweep [dir][hydraulic], 0 = the
+    // default rule; set by hd_autotune (results do not depend on it)
+    int cr_threads[2][2] = {{0, 0}, {0, 0}};
     bool use_graph = false;
     bool capturing = false;
     std::map<int, hipGraphExec_t> graphs;

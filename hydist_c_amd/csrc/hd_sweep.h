@@ -1,6 +1,8 @@
 // hd_sweep.h - buffers and entry of the implicit line sweeps (hd_sweep.hip).
 #pragma once
 
+#include <string>
+
 #include "hd_device.h"
 
 struct hd_ctx;
@@ -32,6 +34,7 @@ struct SweepBufs {
 };
 
 int sweep_rows(const DevState &s, int dir, bool hyd);
+int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report);
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
                    int end, double t, double s_start);
 

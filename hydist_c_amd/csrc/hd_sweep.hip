@@ -894,7 +894,8 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     } else {
         if (!kRegisterCR) {
             const int nz = hyd ? nrows / 2 : nrows;
-            const int threads = nz > 1024 ? 512 : 256;
+            const int tuned = ctx->cr_threads[DIR][hyd ? 1 : 0];
+            const int threads = tuned ? tuned : (nz > 1024 ? 512 : 256);
             if (form == HD_FORM_FP32_REFINE) {
                 const size_t sm = 6 * (size_t)nz * sizeof(float);
                 if (n != 1) return HD_E_UNSUPPORTED;
@@ -958,6 +959,83 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         LAUNCH(ctx, DIR ? "k_fs_extract<j>" : "k_fs_extract<i>",
                (k_fs_extract<DIR, LAYOUT><<<g.grid, g.block, 0, st>>>(S, B, start, end)));
     return HD_OK;
+}
+
+// Times the cyclic-reduction solve of one sweep for each workgroup size
+// and keeps the fastest.  The system is assembled once from the current
+// state; only sweep scratch is written, and every workgroup size gives the
+// same result (each row's update is independent of which lane does it).
+template <int DIR>
+static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
+                     double t, double s_start, std::string &report) {
+    hipStream_t st = ctx->stream;
+    SweepBufs B = ctx->sweep;
+    const int nrows = sweep_rows(s, DIR, hyd);
+    B.ld = B.ld_lm[DIR];
+    const dim3 tg = tile_grid(s, n);
+    if (hyd)
+        LAUNCH(ctx, "k_hyd_assemble", (k_hyd_assemble<DIR, LINE_MAJOR><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end)));
+    else
+        LAUNCH(ctx, "k_fs_assemble",
+               (k_fs_assemble<DIR, LINE_MAJOR><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, 0)));
+    const int lines = end - start + 1, nz = hyd ? nrows / 2 : nrows;
+    const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
+    const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return HD_E_HIP;
+    int best = 0;
+    float best_ms = 0;
+    char buf[160];
+    for (int threads : {128, 256, 512}) {
+        int rc = HD_OK;
+        auto run = [&]() {
+            if (f32) {
+                if (hyd) k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+                else k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+            } else {
+                if (hyd) k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+                else k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+            }
+            rc = check_launch(ctx, "k_cr (tuning)");
+        };
+        if ((rc = allow_lds(ctx, f32 ? (hyd ? k_cr<float, true, true> : k_cr<float, true, false>)
+                                      : (hyd ? k_cr<double, false, true> : k_cr<double, false, false>),
+                            sm)))
+            break;
+        run();  // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < 3 && !rc; r++) run();
+        (void)hipEventRecord(e1, st);
+        if (rc || hipEventSynchronize(e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc ? rc : HD_E_HIP;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= 3;
+        std::snprintf(buf, sizeof(buf), "%s\"%s%s/%d\": %.4f", report.size() > 1 ? ", " : "", DIR ? "uz" : "vz",
+                      hyd ? "" : "_fs", threads, ms);
+        report += buf;
+        if (!best || ms < best_ms) {
+            best = threads;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->cr_threads[DIR][hyd ? 1 : 0] = best;
+    return HD_OK;
+}
+
+int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report) {
+    if (ctx->cfg.form == HD_FORM_FP64_PLAIN || kRegisterCR || n != 1) return HD_OK;  // Thomas: nothing to tune
+    const hd_options &o = ctx->ops;
+    int rc = tune_impl<1>(ctx, S, s, n, true, 2, s.M, 0.0, 0.0, report);
+    if (!rc) rc = tune_impl<0>(ctx, S, s, n, true, 2, s.N, 0.0, 0.0, report);
+    if (!rc && ctx->cfg.sediment) rc = tune_impl<1>(ctx, S, s, n, false, 3, s.M - 1, o.sediment_start, o.bed_change_start, report);
+    if (!rc && ctx->cfg.sediment) rc = tune_impl<0>(ctx, S, s, n, false, 3, s.N - 1, o.sediment_start, o.bed_change_start, report);
+    return rc;
 }
 
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,

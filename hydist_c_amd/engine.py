@@ -57,6 +57,8 @@ def lib():
         for n in ("hd_batch_copy_out", "hd_batch_copy_in"):
             getattr(L, n).argtypes = [vp, C.c_int, C.c_int, dp, C.c_size_t]
         L.hd_set_graph.argtypes = [vp, C.c_int]
+        L.hd_autotune.argtypes = [vp, C.c_char_p, C.c_size_t]
+        L.hd_set_solver_threads.argtypes = [vp, C.c_int, C.c_int, C.c_int]
         L.hd_snapshot_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.hd_snapshot.argtypes = [vp, C.c_void_p, C.c_size_t, C.c_double]
         L.hd_restore.argtypes = [vp, C.c_void_p, C.c_size_t, dp]
@@ -201,6 +203,18 @@ class Simulation:
         v, q = C.c_double(), C.c_double()
         check(lib().hd_mass_balance(self.h, C.byref(v), C.byref(q)), self.h)
         return v.value, q.value
+
+
+def autotune(sim):
+    """Picks the fastest solver workgroup size per sweep; returns the report."""
+    import json
+    buf = C.create_string_buffer(2048)
+    check(lib().hd_autotune(sim.h, buf, 2048), sim.h)
+    return json.loads(buf.value.decode())
+
+
+def set_solver_threads(sim, direction, hydraulic, threads):
+    check(lib().hd_set_solver_threads(sim.h, direction, int(hydraulic), threads), sim.h)
 
 
 def _stage_api():

@@ -247,6 +247,17 @@ int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t
  * time it is needed.  Results are identical to eager launches. */
 int hd_set_graph(hd_ctx *ctx, int on);
 
+/* Autotuning (no counterpart in the reference).  Times the fast forms'
+ * cyclic-reduction solve of each sweep (UZ, VZ, and the FS sweeps when
+ * sediment is on) at several workgroup sizes on the context's current
+ * state and keeps the fastest; writes a JSON report (ms per candidate and
+ * the choice) into report.  Results do not depend on the choice, so tuning
+ * never changes a run's output.  hd_set_solver_threads sets one choice by
+ * hand (dir 1 = UZ, 0 = VZ; threads a multiple of 64 up to 512, 0 = the
+ * default rule). */
+int hd_autotune(hd_ctx *ctx, char *report, size_t n);
+int hd_set_solver_threads(hd_ctx *ctx, int dir, int hydraulic, int threads);
+
 /* Checkpoint / restart.  Replaces the reference's stubs (the -c/-T
  * options, main.cpp:283-284, loader.cpp:172-185): the snapshot holds every
  * buffer whose contents carry from one step to the next (state, derived
