@@ -127,7 +127,7 @@ def time_form(sc, form, steps, warmup, profile_steps):
     return sim, ms_per_step, stats, finite, tune
 
 
-def cpu_baseline(sc, seconds):
+def cpu_baseline(sc, seconds, threads=None):
     """The oracle (OpenMP over lines) on this host, on the same mesh with
     every stage active (sediment and bed change from t = 0), for a bounded
     number of steps."""
@@ -138,7 +138,8 @@ def cpu_baseline(sc, seconds):
     s2 = copy.copy(sc)
     s2.sediment_start = 0.0
     s2.bed_change_start = 0.0
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     ol.set_threads(threads)
     o = ol.Oracle(s2, cfg=make_config(sediment=True, bed_change=True))
     o.init()
@@ -232,6 +233,9 @@ def main():
         log("[bench] CPU oracle baseline")
         cpu = cpu_baseline(sc, args.cpu_seconds)
         log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s ({cpu['sample']})")
+        cpu1 = cpu_baseline(sc, args.cpu_seconds / 2, threads=1)
+        log(f"[bench] cpu, 1 thread: {cpu1['value']:.3e} cell-steps/s ({cpu1['sample']})")
+        cpu["single_thread"] = {"value": cpu1["value"], "sample": cpu1["sample"]}
     out = {
         "metric": METRIC,
         "value": r["cell_steps_per_s"],
