@@ -156,6 +156,38 @@ def cpu_baseline(sc, seconds, threads=None):
                       f"{dt:.1f} s on {threads} threads"}
 
 
+def serving_measure(seconds, rate, seed=0, max_active=32, side=(128, 512), duration=(60.0, 600.0)):
+    """The serving loop under Poisson arrivals: `rate` requests/s for
+    `seconds` s of what-if runs (meander + islands meshes with sides
+    log-uniform in [128, 512], durations log-uniform in [60, 600] s
+    simulated, sediment on for half, FP32 + refine for half), then waits
+    for them to finish.  Reports p50/p99 completion latency and the
+    aggregate cell-steps/s of completed runs."""
+    import numpy as np
+    from hydist_c_amd import engine, scenarios as S
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 10))
+    arrivals = arrivals[arrivals < seconds]
+    reqs = [S.serving_request(k, side=side, duration=duration) for k in range(len(arrivals))]
+    srv = engine.Server(hbm_budget=64 << 30, max_active=max_active, steps_per_quantum=8, use_graph=False)
+    t0 = time.perf_counter()
+    ids = []
+    for at, (sc, T, form, sed) in zip(arrivals, reqs):
+        now = time.perf_counter() - t0
+        if at > now:
+            time.sleep(at - now)
+        ids.append(srv.submit(sc, T, form=form, sediment=sed, bed_change=sed))
+    deadline = time.perf_counter() + 120.0
+    for rid in ids:
+        srv.wait(rid, timeout=max(0.0, deadline - time.perf_counter()))
+    st = srv.stats()
+    srv.close()
+    st.update({"rate_per_s": rate, "window_s": seconds,
+               "workload": f"meander+islands, side log-U[{side[0]},{side[1]}], duration log-U[{duration[0]:g},"
+                           f"{duration[1]:g}] s simulated, sediment p=0.5, FP32+refine p=0.5"})
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +197,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=10.0,
+                    help="arrival window of the serving measurement (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--sediment-from-start", action="store_true",
                     help="switch sediment and bed change on at t=0 (short profiling runs)")
     args = ap.parse_args()
@@ -228,6 +263,12 @@ def main():
     step_bytes = STEP_BYTES["bed_change"] * sc.M * sc.N
     step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9}
     step_roof["frac"] = step_roof["achieved_GBs"] / HBM_PEAK_GBS
+    serving = None
+    if args.serving_seconds > 0:
+        log(f"[bench] serving loop: {args.serving_rate:.0f} req/s for {args.serving_seconds:.0f} s")
+        serving = serving_measure(args.serving_seconds, args.serving_rate)
+        log(f"[bench] serving: p50 {serving['p50_s']:.3f} s, p99 {serving['p99_s']:.3f} s, "
+            f"{serving['completed']}/{serving['submitted']} done, {serving['cell_steps_per_s']:.3e} cell-steps/s")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
@@ -260,6 +301,7 @@ def main():
         "step_roofline": step_roof,
         "tridiag_roofline": tri,
         "cpu_baseline": cpu,
+        "serving": serving,
         "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / v[0]} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],
     }

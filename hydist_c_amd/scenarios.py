@@ -431,3 +431,23 @@ def tidal_flats(M, N, seed=6, shelf=(0.03, 0.16), channel_depth=3.0, drop=2.0, r
         boundary_type=(0, 0, 0, 0), bc=[zeros, np.ascontiguousarray(z_down), zeros, zeros], CC=None,
         meta={"seed": seed},
     )
+
+
+def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0.5, p_fp32=0.5):
+    """What-if request k of the serving workload: a meander + islands mesh
+    with a log-uniform side in `side` (the other side 0.5-1.0 of it), a
+    log-uniform simulated duration in `duration` (s), sediment on with
+    probability p_sediment, and FP32 + refine (the looser tolerance) with
+    probability p_fp32, else plain FP64.  Returns (scenario, Tmax, form,
+    sediment)."""
+    rng = np.random.default_rng(50_000 + k)
+    M = int(round(np.exp(rng.uniform(np.log(side[0]), np.log(side[1])))))
+    N = max(32, int(round(M * rng.uniform(0.5, 1.0))))
+    T = float(np.exp(rng.uniform(np.log(duration[0]), np.log(duration[1]))))
+    T = 2.0 * np.ceil(T / 2.0)
+    sed = bool(rng.random() < p_sediment)
+    form = 2 if rng.random() < p_fp32 else 0
+    n_isl = max(1, int(30 * M * N / (2048 * 1024)))
+    s = meander_islands(M, N, seed=k, n_islands=n_isl, Q=-4000.0 * N / 1024, hours=int(np.ceil(T / 3600.0)),
+                        sediment_start=0.0, name=f"request{k}_{M}x{N}")
+    return s, T, form, sed
