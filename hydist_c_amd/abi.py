@@ -142,6 +142,8 @@ def make_options(scen, Tmax=3600.0):
     o.debug = False
     o.plot = False
     o.total_time = scen.total_time
+    for k, v in getattr(scen, "options", {}).items():
+        setattr(o, k, v)
     return o
 
 

@@ -42,6 +42,9 @@ class Scenario:
     z0: np.ndarray = None
     FS0: np.ndarray = None
     meta: dict = field(default_factory=dict)
+    # Options members to override (e.g. the narrow-channel modes:
+    # {"channel": True, "kenhhepd": True}, engine.cpp:24-30,77-80,190-193)
+    options: dict = field(default_factory=dict)
 
     @property
     def shape(self):

@@ -146,3 +146,17 @@ def test_mass_balance_gpu():
     assert q == 0.0
     assert abs(v1 - v0) < 1e-9 * v0
     assert abs(v1 - vo) < 1e-12 * vo
+
+
+@pytest.mark.parametrize("mode", ["kenhhepd", "kenhhepng"])
+def test_narrow_channel_modes_bitwise(mode):
+    """Channel mode (engine.cpp:24-30,77-80,190-193): no boundary update,
+    and the narrowed sweep range of the chosen direction."""
+    _print_case("channel", mode=mode)
+    sc = S.meander_channel(96, 48, seed=7, amplitude=6.0, wavelength=96.0, half_width=12.0)
+    sc.options = {"channel": True, mode: True}
+    g, o = make_pair(sc)
+    g.step(15)
+    o.step(15)
+    rep = diff_report(g, o, HYDRO)
+    assert all(v["bitwise"] for v in rep.values()), rep
