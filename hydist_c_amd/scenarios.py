@@ -352,7 +352,10 @@ def braided(M, N, seed=5, n_channels=12, width=(10, 120), depth=(1.0, 10.0), Q=-
 
 
 def config3(seed=5):
-    return braided(4096, 4096, seed=seed, name="config3_4096x4096")
+    """4096 x 4096 braided river (12 random-walk channels), dt = 1 s."""
+    s = braided(4096, 4096, seed=seed, name="config3_4096x4096")
+    s.coeff_overrides = dict(s.coeff_overrides, dT=1.0)
+    return s
 
 
 def ensemble_member(k, M=512, N=256):

@@ -1,4 +1,5 @@
 """One simulated hour of the 2048 x 1024 meander + islands workload
+(or, with `c3`, 600 s of the 4096 x 4096 braided river at dt = 1 s)
 (sediment and FP64 bed change from t = 300 s, dt = 2 s, 1800 steps) in each
 GPU form, compared with the CPU oracle after the hour: max |dZ|, |dU|,
 |dV|, |dFS|, |d dH| over cells wet in both, and the wet-mask difference.
@@ -30,13 +31,21 @@ def log(*a):
 
 
 def main():
-    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 1800
-    sc = S.config2() if len(sys.argv) < 3 else S.meander_islands(int(sys.argv[2]), int(sys.argv[3]), seed=2)
+    # arguments: [steps] [M N] | c3 [steps]
+    if len(sys.argv) > 1 and sys.argv[1] == "c3":
+        # 4096 x 4096 braided river, dt = 1 s, 600 s, hydraulics only
+        sc = S.config3()
+        steps = int(sys.argv[2]) if len(sys.argv) > 2 else 600
+        sed = False
+    else:
+        steps = int(sys.argv[1]) if len(sys.argv) > 1 else 1800
+        sc = S.config2() if len(sys.argv) < 3 else S.meander_islands(int(sys.argv[2]), int(sys.argv[3]), seed=2)
+        sed = True
     log(f"[acc] mesh {sc.M}x{sc.N}, {steps} steps")
     sims = {}
     for name, form in FORMS.items():
         t0 = time.time()
-        g = engine.Simulation(sc, form=form, sediment=True, bed_change=True)
+        g = engine.Simulation(sc, form=form, sediment=sed, bed_change=sed)
         g.init()
         for k in range(0, steps, 300):
             g.step(min(300, steps - k))
@@ -44,9 +53,9 @@ def main():
         sims[name] = g
         log(f"[acc] gpu {name}: {time.time() - t0:.1f} s, t = {g.t}")
     # the oracle seeded with the GPU's pow/log-derived init fields
-    o = ol.Oracle(sc, cfg=ol.make_config(sediment=True, bed_change=True))
+    o = ol.Oracle(sc, cfg=ol.make_config(sediment=sed, bed_change=sed))
     o.init()
-    ref = engine.Simulation(sc, form=abi.HD_FORM_FP64_PLAIN, sediment=True, bed_change=True)
+    ref = engine.Simulation(sc, form=abi.HD_FORM_FP64_PLAIN, sediment=sed, bed_change=sed)
     ref.init()
     for f in ("KX1", "KY1"):
         o.set(f, ref.get(f))
@@ -70,7 +79,8 @@ def main():
             row[f"bitwise_{f}"] = bool(bits_equal(a, b))
         table[name] = row
         log(f"[acc] {name}: {row}")
-    print(json.dumps({"steps": steps, "simulated_s": steps * 2.0, "mesh": [sc.M, sc.N], "forms": table}), flush=True)
+    print(json.dumps({"steps": steps, "simulated_s": steps * sims["plain"].coeffs.dT, "mesh": [sc.M, sc.N],
+                      "forms": table}), flush=True)
 
 
 if __name__ == "__main__":
