@@ -89,6 +89,20 @@ def stage_bytes(name, sim, sc):
     return None
 
 
+def measured_traffic(symbol):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "c2_2048x1024_fast_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch "):
+            name, value = line[len("traffic_bytes_per_launch "):].rsplit(" ", 1)
+            if name.strip() == symbol:
+                return float(value)
+    return None
+
+
 def kernel_symbol(stage):
     """The kernel (as rocprofv3 names it) behind a stage name."""
     if stage.startswith("k_cr<"):
@@ -248,9 +262,12 @@ def main():
     if known:
         nbytes = total_bytes / launches
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
+        traffic = measured_traffic(dom_name)
         roofline = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                     "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms, "launches": launches}
+        if traffic is not None:
+            roofline["traffic_source"] = "profiles/c2_2048x1024_fast_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     # the tridiagonal solve kernel of the best form, at its own roofline
     tri = None
     solver = [(k, v) for k, v in stats.items() if k.startswith(("k_cr", "k_thomas")) and "fs" not in k]
