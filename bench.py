@@ -248,6 +248,8 @@ def main():
     # stages, e.g. one cyclic-reduction kernel for both hydraulic sweeps)
     groups = {}
     for name, (launches, total_ms) in stats.items():
+        if launches == 0 or total_ms <= 0:
+            continue
         g = groups.setdefault(kernel_symbol(name), [0, 0.0, 0.0, True])
         b = stage_bytes(name, sim, sc)
         g[0] += launches
@@ -256,10 +258,11 @@ def main():
             g[3] = False
         else:
             g[2] += b * launches
-    dom_name, (launches, total_ms, total_bytes, known) = max(groups.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / launches
     roofline = None
-    if known:
+    if groups:
+        dom_name, (launches, total_ms, total_bytes, known) = max(groups.items(), key=lambda kv: kv[1][1])
+        avg_ms = total_ms / launches
+    if groups and known:
         nbytes = total_bytes / launches
         achieved = nbytes / (avg_ms * 1e-3) / 1e9
         traffic = measured_traffic(dom_name)
@@ -270,7 +273,8 @@ def main():
             roofline["traffic_source"] = "profiles/c2_2048x1024_fast_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     # the tridiagonal solve kernel of the best form, at its own roofline
     tri = None
-    solver = [(k, v) for k, v in stats.items() if k.startswith(("k_cr", "k_thomas")) and "fs" not in k]
+    solver = [(k, v) for k, v in stats.items()
+              if k.startswith(("k_cr", "k_thomas")) and "fs" not in k and v[0] > 0 and v[1] > 0]
     if solver:
         k, (launches, total_ms) = max(solver, key=lambda kv: kv[1][1])
         b = stage_bytes(k, sim, sc)
@@ -311,7 +315,7 @@ def main():
                    "M": sc.M, "N": sc.N, "form": best, "timed_from_t": sc.sediment_start},
         "forms": {k: {"cell_steps_per_s": v["cell_steps_per_s"], "ms_per_step": v["ms_per_step"],
                       "finite": v["finite"], "solver_threads": v["tune"]["chosen"] if v["tune"] else None,
-                      "profile_top": sorted(({"stage": a, "launches": b[0], "avg_ms": b[1] / b[0]}
+                      "profile_top": sorted(({"stage": a, "launches": b[0], "avg_ms": b[1] / max(b[0], 1)}
                                              for a, b in v["stats"].items()), key=lambda d: -d["avg_ms"])[:6]}
                   for k, v in results.items()},
         "roofline": roofline,
@@ -319,7 +323,7 @@ def main():
         "tridiag_roofline": tri,
         "cpu_baseline": cpu,
         "serving": serving,
-        "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / v[0]} for k, v in stats.items()),
+        "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / max(v[0], 1)} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],
     }
     print(json.dumps(out), flush=True)
