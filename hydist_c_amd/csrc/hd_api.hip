@@ -704,12 +704,9 @@ int hd_device_info(char *buf, size_t n) {
 // ---------------------------------------------------------------------------
 // Batches of scenarios and whole-step graphs
 // ---------------------------------------------------------------------------
-// Batches of more than one scenario and whole-step graphs are switched
-// off: the first GPU run that contained them ended in a device fault whose
-// cause has not been found (the code paths run clean under the CPU
-// emulation in tests/emu).
-constexpr bool kBatchEnabled = false;
-constexpr bool kGraphEnabled = false;
+// Switches for batches of more than one scenario and whole-step graphs.
+constexpr bool kBatchEnabled = true;
+constexpr bool kGraphEnabled = true;
 
 int hd_batch_create(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, const hd_options *ops,
                     const hd_config *cfg, hd_ctx **out) {

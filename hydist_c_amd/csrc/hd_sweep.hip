@@ -560,9 +560,10 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
 
 // The FP64 reduced rows (for the residual) are kept in the Ap/Bp scratch.
 template <typename T, bool REFINE, bool INTERLEAVED>
-__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs B, int start, int end,
+__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs Bin, int start, int end,
                                            int nrows) {
     extern __shared__ unsigned char smem[];
+    const SweepBufs B = Bin.scen(blockIdx.y);
     const int line = start + blockIdx.x;
     if (line > end) return;
     const int ld = B.ld;
@@ -898,26 +899,24 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
             const int threads = tuned ? tuned : (nz > 1024 ? 512 : 256);
             if (form == HD_FORM_FP32_REFINE) {
                 const size_t sm = 6 * (size_t)nz * sizeof(float);
-                if (n != 1) return HD_E_UNSUPPORTED;
                 int rc = allow_lds(ctx, hyd ? k_cr<float, true, true> : k_cr<float, true, false>, sm);
                 if (rc) return rc;
                 if (hyd)
                     LAUNCH(ctx, DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>",
-                           (k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                           (k_cr<float, true, true><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
                 else
                     LAUNCH(ctx, DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>",
-                           (k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                           (k_cr<float, true, false><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
             } else {
                 const size_t sm = 4 * (size_t)nz * sizeof(double);
-                if (n != 1) return HD_E_UNSUPPORTED;
                 int rc = allow_lds(ctx, hyd ? k_cr<double, false, true> : k_cr<double, false, false>, sm);
                 if (rc) return rc;
                 if (hyd)
                     LAUNCH(ctx, DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>",
-                           (k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                           (k_cr<double, false, true><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
                 else
                     LAUNCH(ctx, DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>",
-                           (k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows)));
+                           (k_cr<double, false, false><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
             }
         } else {
             const int nz = hyd ? nrows / 2 : nrows;

@@ -14,8 +14,6 @@ from parity_util import bits_equal
 
 pytestmark = pytest.mark.gpu
 
-OFF = ("batched contexts and whole-step graphs are switched off in the library: the first GPU run "
-       "containing them ended in a device fault whose cause is not found yet")
 
 FIELDS = ["U", "V", "Z", "FS", "DH", "KHOUOT"]
 FORMS = {"plain": abi.HD_FORM_FP64_PLAIN, "fast": abi.HD_FORM_FP64_FAST, "fp32": abi.HD_FORM_FP32_REFINE}
@@ -34,7 +32,6 @@ def _case(name, **kw):
     print(f"[case] {name} {kw}", flush=True)
 
 
-@pytest.mark.skip(reason=OFF)
 @pytest.mark.parametrize("form", ["plain", "fast", "fp32"])
 def test_batch_equals_single_runs(form):
     _case("batch vs single", form=form)
@@ -54,7 +51,6 @@ def test_batch_equals_single_runs(form):
     b.close()
 
 
-@pytest.mark.skip(reason=OFF)
 @pytest.mark.parametrize("form", ["plain", "fast"])
 def test_graph_replays_equal_eager(form):
     _case("graph vs eager", form=form)
@@ -76,18 +72,6 @@ def test_graph_replays_equal_eager(form):
     graph.close()
 
 
-def test_switched_off_paths_raise():
-    _case("switched off")
-    scens = [S.ensemble_member(k, M=64, N=32) for k in range(2)]
-    with pytest.raises(engine.HydistError, match="switched off"):
-        engine.Batch(scens)
-    b = engine.Batch(scens[:1])
-    with pytest.raises(engine.HydistError, match="switched off"):
-        engine.lib().hd_set_graph(b.h, 1) == 0 or engine.check(engine.lib().hd_set_graph(b.h, 1), b.h)
-    b.close()
-
-
-@pytest.mark.skip(reason=OFF)
 def test_batch_rejects_mixed_shapes():
     _case("batch shapes")
     a = S.ensemble_member(0, M=64, N=32)
