@@ -103,8 +103,17 @@ def measured_traffic(symbol):
     return None
 
 
-def kernel_symbol(stage):
+def kernel_symbol(stage, sc=None):
     """The kernel (as rocprofv3 names it) behind a stage name."""
+    if stage.startswith("k_cr_reg<") and sc is not None:
+        prec, what = stage[9:-1].split(",")
+        t = "double, false" if prec == "f64" else "float, true"
+        # rows per thread: ceil(n / threads) rounded up to 2, 4 or 8
+        n = (sc.N - 1 if what in ("uz", "fs_j") else sc.M - 1)
+        threads = 512 if n > 1024 else 256
+        kpt = -(-n // threads)
+        k = 2 if kpt <= 2 else 4 if kpt <= 4 else 8
+        return f"k_cr_reg<{t}, {'false' if what.startswith('fs') else 'true'}, {k}>"
     if stage.startswith("k_cr<"):
         prec, what = stage[5:-1].split(",")
         t = "double, false" if prec == "f64" else "float, true"
@@ -250,7 +259,7 @@ def main():
     for name, (launches, total_ms) in stats.items():
         if launches == 0 or total_ms <= 0:
             continue
-        g = groups.setdefault(kernel_symbol(name), [0, 0.0, 0.0, True])
+        g = groups.setdefault(kernel_symbol(name, sc), [0, 0.0, 0.0, True])
         b = stage_bytes(name, sim, sc)
         g[0] += launches
         g[1] += total_ms

@@ -860,7 +860,7 @@ int sweep_rows(const DevState &s, int dir, bool hyd) {
 // The register-resident solver (k_cr_reg) is switched off: the first GPU
 // run that contained it (with batched contexts and whole-step graphs) ended
 // in a device fault whose cause has not been found.
-constexpr bool kRegisterCR = false;
+constexpr bool kRegisterCR = true;
 
 template <typename K>
 static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
@@ -920,7 +920,9 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
             }
         } else {
             const int nz = hyd ? nrows / 2 : nrows;
-            const int threads = nz > 1024 ? 512 : 256;
+            const int tuned = ctx->cr_threads[DIR][hyd ? 1 : 0];
+            int threads = nz > 1024 ? 512 : 256;
+            if (tuned && (nz + tuned - 1) / tuned <= 8) threads = tuned;
             const int kpt = (nz + threads - 1) / threads;
             if (kpt > 8) {
                 ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the solver's 4096";
@@ -987,20 +989,48 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     char buf[160];
     for (int threads : {128, 256, 512}) {
         int rc = HD_OK;
-        auto run = [&]() {
+        const int kpt = (nz + threads - 1) / threads;
+        if (kRegisterCR && kpt > 8) continue;
+        auto reg = [&](auto kp) {
+            constexpr int K = decltype(kp)::value;
             if (f32) {
+                if (hyd) k_cr_reg<float, true, true, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+                else k_cr_reg<float, true, false, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+            } else {
+                if (hyd) k_cr_reg<double, false, true, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+                else k_cr_reg<double, false, false, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
+            }
+        };
+        auto run = [&]() {
+            if (kRegisterCR) {
+                if (kpt <= 2) reg(std::integral_constant<int, 2>{});
+                else if (kpt <= 4) reg(std::integral_constant<int, 4>{});
+                else reg(std::integral_constant<int, 8>{});
+            } else if (f32) {
                 if (hyd) k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
                 else k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
             } else {
                 if (hyd) k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
                 else k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
             }
-            rc = check_launch(ctx, "k_cr (tuning)");
+            rc = check_launch(ctx, "cyclic reduction (tuning)");
         };
-        if ((rc = allow_lds(ctx, f32 ? (hyd ? k_cr<float, true, true> : k_cr<float, true, false>)
-                                      : (hyd ? k_cr<double, false, true> : k_cr<double, false, false>),
-                            sm)))
-            break;
+        if (sm > 65536) {
+            // the LDS attribute of every variant this tuner may launch
+            for (const void *k : {(const void *)k_cr<float, true, true>, (const void *)k_cr<float, true, false>,
+                                  (const void *)k_cr<double, false, true>, (const void *)k_cr<double, false, false>,
+                                  (const void *)k_cr_reg<float, true, true, 8>, (const void *)k_cr_reg<float, true, false, 8>,
+                                  (const void *)k_cr_reg<double, false, true, 8>, (const void *)k_cr_reg<double, false, false, 8>,
+                                  (const void *)k_cr_reg<float, true, true, 4>, (const void *)k_cr_reg<float, true, false, 4>,
+                                  (const void *)k_cr_reg<double, false, true, 4>, (const void *)k_cr_reg<double, false, false, 4>,
+                                  (const void *)k_cr_reg<float, true, true, 2>, (const void *)k_cr_reg<float, true, false, 2>,
+                                  (const void *)k_cr_reg<double, false, true, 2>, (const void *)k_cr_reg<double, false, false, 2>})
+                if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess) {
+                    rc = HD_E_HIP;
+                    break;
+                }
+            if (rc) break;
+        }
         run();  // warm-up
         (void)hipEventRecord(e0, st);
         for (int r = 0; r < 3 && !rc; r++) run();
@@ -1028,7 +1058,7 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
 }
 
 int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report) {
-    if (ctx->cfg.form == HD_FORM_FP64_PLAIN || kRegisterCR || n != 1) return HD_OK;  // Thomas: nothing to tune
+    if (ctx->cfg.form == HD_FORM_FP64_PLAIN || n != 1) return HD_OK;  // Thomas: nothing to tune
     const hd_options &o = ctx->ops;
     int rc = tune_impl<1>(ctx, S, s, n, true, 2, s.M, 0.0, 0.0, report);
     if (!rc) rc = tune_impl<0>(ctx, S, s, n, true, 2, s.N, 0.0, 0.0, report);
