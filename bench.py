@@ -211,6 +211,31 @@ def serving_measure(seconds, rate, seed=0, max_active=32, side=(128, 512), durat
     return st
 
 
+def ensemble_measure(n_scen, steps, forms=("plain", "fast", "fp32")):
+    """The ensemble workload: n_scen independent 512 x 256 meandering
+    channels (per-scenario inflow amplitude, tide and Manning factor) in
+    one batched context, every stage one launch over all of them and each
+    step one graph launch.  Reports cell-steps/s per form over `steps`
+    steps after 5 warm-up steps."""
+    from hydist_c_amd import abi, engine, scenarios as S
+    ids = {"plain": abi.HD_FORM_FP64_PLAIN, "fast": abi.HD_FORM_FP64_FAST, "fp32": abi.HD_FORM_FP32_REFINE}
+    scens = [S.ensemble_member(k) for k in range(n_scen)]
+    out = {"scenarios": n_scen, "mesh": [scens[0].M, scens[0].N], "steps": steps, "graph": True}
+    for f in forms:
+        b = engine.Batch(scens, form=ids[f], graph=True)
+        b.init()
+        b.step(5)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        dt = time.perf_counter() - t0
+        cells = n_scen * scens[0].M * scens[0].N
+        out[f] = {"ms_per_step": dt * 1e3 / steps, "cell_steps_per_s": cells * steps / dt}
+        b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,6 +248,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving measurement (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--ensemble", type=int, default=256, help="scenarios of the ensemble measurement (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=30)
     ap.add_argument("--sediment-from-start", action="store_true",
                     help="switch sediment and bed change on at t=0 (short profiling runs)")
     args = ap.parse_args()
@@ -293,6 +320,13 @@ def main():
     step_bytes = STEP_BYTES["bed_change"] * sc.M * sc.N
     step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9}
     step_roof["frac"] = step_roof["achieved_GBs"] / HBM_PEAK_GBS
+    ensemble = None
+    if args.ensemble > 0:
+        log(f"[bench] ensemble: {args.ensemble} x 512x256 in one batched context, {args.ensemble_steps} steps per form")
+        ensemble = ensemble_measure(args.ensemble, args.ensemble_steps)
+        log("[bench] ensemble: " + ", ".join(f"{f} {ensemble[f]['ms_per_step']:.2f} ms/step "
+                                             f"{ensemble[f]['cell_steps_per_s']:.3e} cell-steps/s"
+                                             for f in ("plain", "fast", "fp32")))
     serving = None
     if args.serving_seconds > 0:
         log(f"[bench] serving loop: {args.serving_rate:.0f} req/s for {args.serving_seconds:.0f} s")
@@ -332,6 +366,7 @@ def main():
         "tridiag_roofline": tri,
         "cpu_baseline": cpu,
         "serving": serving,
+        "ensemble": ensemble,
         "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / max(v[0], 1)} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],
     }
