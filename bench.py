@@ -248,7 +248,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving measurement (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=20.0)
-    ap.add_argument("--ensemble", type=int, default=256, help="scenarios of the ensemble measurement (0 = skip)")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="scenarios of the ensemble measurement, e.g. 256 (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=30)
     ap.add_argument("--sediment-from-start", action="store_true",
                     help="switch sediment and bed change on at t=0 (short profiling runs)")

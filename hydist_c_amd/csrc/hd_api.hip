@@ -41,12 +41,15 @@ int set_err(hd_ctx *ctx, int code, const std::string &msg) {
 
 namespace hd {
 
+// Allocates and zero-fills on the context's stream, so the fill is ordered
+// before every later upload and kernel on that stream (hipMemset would run
+// on the null stream, which a non-blocking stream does not wait for).
 int ctx_alloc(hd_ctx *ctx, void **p, size_t bytes) {
     void *q = nullptr;
     if (bytes == 0) bytes = 8;
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess) return set_err(ctx, HD_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(q, 0, bytes);
+    e = hipMemsetAsync(q, 0, bytes, ctx->stream);
     if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     ctx->allocs.push_back(q);
     ctx->bytes += bytes;
@@ -339,6 +342,21 @@ int hd_load_coeffs(hd_coeffs *ce) {
     return hd_finalize_coeffs(ce);
 }
 
+// Reads the device descriptors back and compares them with the host copies,
+// so a descriptor that did not arrive intact is an error before any kernel
+// reads it.
+static int verify_desc(hd_ctx *ctx) {
+    std::vector<DevState> back(ctx->nscen);
+    if (hipMemcpy(back.data(), ctx->ds, back.size() * sizeof(DevState), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_err(ctx, HD_E_HIP, "descriptor read-back failed");
+    for (int k = 0; k < ctx->nscen; k++) {
+        const DevState &h = k == 0 ? ctx->hs : ctx->hss[k - 1];
+        if (std::memcmp(&back[k], &h, sizeof(DevState)) != 0)
+            return set_err(ctx, HD_E_HIP, "device descriptor " + std::to_string(k) + " does not match the host copy");
+    }
+    return HD_OK;
+}
+
 static int check_mesh(const hd_mesh *mesh) {
     if (mesh->M < 4 || mesh->N < 4 || !mesh->h || !mesh->hsnham || !mesh->VISCOIDX)
         return set_err(nullptr, HD_E_INVALID, "mesh needs M, N >= 4 and h, hsnham, VISCOIDX");
@@ -394,6 +412,7 @@ static int create_ctx(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, con
     }
     if (!rc) rc = hd::alloc_shared(ctx, n);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = set_err(ctx, HD_E_HIP, "upload failed");
+    if (!rc) rc = verify_desc(ctx);
     if (rc) {
         g_err = ctx->err;
         hd_destroy(ctx);

@@ -17,8 +17,9 @@ int round_up(int x, int m) { return (x + m - 1) / m * m; }
 int upload2d(hd_ctx *ctx, double *dst, const double *src, const DevState &s) {
     if (!src) return HD_OK;
     const size_t W = s.M + 3, H = s.N + 3;
-    hipError_t e = hipMemcpy2D(dst, s.P * sizeof(double), src, W * sizeof(double), W * sizeof(double), H,
-                               hipMemcpyHostToDevice);
+    // on the context's stream, after the buffer's zero-fill
+    hipError_t e = hipMemcpy2DAsync(dst, s.P * sizeof(double), src, W * sizeof(double), W * sizeof(double), H,
+                                    hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) {
         ctx->err = std::string("upload: ") + hipGetErrorString(e);
         return HD_E_HIP;
@@ -97,11 +98,11 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     if ((rc = upload2d(ctx, s.z, mesh->z0, s))) return rc;
     if ((rc = upload2d(ctx, s.FS, mesh->FS0, s))) return rc;
     for (int k = 0; k < 4; k++) {
-        if (hipMemcpy((void *)s.bc[k], mesh->bc[k], s.total_time * sizeof(double), hipMemcpyHostToDevice) !=
-            hipSuccess)
+        if (hipMemcpyAsync((void *)s.bc[k], mesh->bc[k], s.total_time * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream) != hipSuccess)
             return HD_E_HIP;
-        if (mesh->CC[k] && hipMemcpy((void *)s.CC[k], mesh->CC[k], s.total_time * sizeof(double),
-                                     hipMemcpyHostToDevice) != hipSuccess)
+        if (mesh->CC[k] && hipMemcpyAsync((void *)s.CC[k], mesh->CC[k], s.total_time * sizeof(double),
+                                          hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
             return HD_E_HIP;
     }
     return HD_OK;
