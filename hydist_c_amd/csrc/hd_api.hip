@@ -530,6 +530,62 @@ int hd_run(hd_ctx *ctx, double *t, double Tmax) {
     return HD_OK;
 }
 
+int hd_run_cb(hd_ctx *ctx, double *t, double Tmax, int interval, hd_output_cb cb, void *user) {
+    if (!ctx || !t || interval < 1) return set_err(ctx, HD_E_INVALID, "ctx/t NULL or interval < 1");
+    const int T = (int)Tmax;
+    while (*t < T) {
+        int rc = hd_step(ctx, t);
+        if (rc) return rc;
+        if (cb && ((int)*t % interval == 0) && (*t - (int)*t == 0)) {
+            HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (cb(ctx, *t, user)) return set_err(ctx, HD_E_CANCELLED, "stopped by the output callback");
+        }
+    }
+    return HD_OK;
+}
+
+static int stage_guard(hd_ctx *ctx) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    return HD_OK;
+}
+
+int hd_sediment_j(hd_ctx *ctx, double t) {
+    int rc = stage_guard(ctx);
+    if (rc) return rc;
+    const DevState &s = ctx->hs;
+    const int n = ctx->nscen;
+    Launch2D g = grid2d(s, n);
+    LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    rc = run_line_sweep(ctx, ctx->ds, s, n, 1, false, 3, s.M - 1, t, ctx->ops.bed_change_start);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    return HD_OK;
+}
+
+int hd_sediment_i(hd_ctx *ctx, double t) {
+    int rc = stage_guard(ctx);
+    if (rc) return rc;
+    const DevState &s = ctx->hs;
+    const int n = ctx->nscen;
+    Launch2D g = grid2d(s, n);
+    LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    rc = run_line_sweep(ctx, ctx->ds, s, n, 0, false, 3, s.N - 1, t, ctx->ops.bed_change_start);
+    if (rc) return rc;
+    LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    return HD_OK;
+}
+
+int hd_bedload(hd_ctx *ctx, double t) {
+    (void)t;
+    int rc = stage_guard(ctx);
+    if (rc) return rc;
+    Launch2D g = grid2d(ctx->hs, ctx->nscen);
+    LAUNCH(ctx, "k_qb", k_qb<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    LAUNCH(ctx, "k_bedload", k_bedload<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    return HD_OK;
+}
+
 int hd_sync(hd_ctx *ctx) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -36,7 +36,8 @@ def lib():
                                 C.POINTER(abi.Config), C.POINTER(vp)]
         for n in ("hd_init", "hd_sync", "hd_destroy"):
             getattr(L, n).argtypes = [vp]
-        for n in ("hd_update_boundary", "hd_halfstep_uz", "hd_halfstep_vz"):
+        for n in ("hd_update_boundary", "hd_halfstep_uz", "hd_halfstep_vz", "hd_sediment_j", "hd_sediment_i",
+                  "hd_bedload"):
             getattr(L, n).argtypes = [vp, C.c_double]
         L.hd_step.argtypes = [vp, dp]
         L.hd_run.argtypes = [vp, dp, C.c_double]
@@ -131,6 +132,29 @@ class Simulation:
 
     def halfstep_uz(self, t):
         check(lib().hd_halfstep_uz(self.h, t), self.h)
+
+    def stage(self, name, t):
+        """Stage block: 'sediment_j', 'sediment_i' or 'bedload' (hd_sediment_j, ...)."""
+        check(getattr(lib(), "hd_" + name)(self.h, t), self.h)
+
+    def run(self, Tmax, interval=None, callback=None):
+        """hd_run / hd_run_cb: steps until t >= int(Tmax); with a callback,
+        callback(t) runs at every whole multiple of `interval` seconds and
+        returning True stops the run."""
+        t = C.c_double(self.t)
+        if callback is None:
+            check(lib().hd_run(self.h, C.byref(t), Tmax), self.h)
+        else:
+            CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
+            cb = CB(lambda h, tt, u: int(bool(callback(tt))))
+            lib().hd_run_cb.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_double, C.c_int, CB, C.c_void_p]
+            rc = lib().hd_run_cb(self.h, C.byref(t), Tmax, interval, cb, None)
+            self.t = t.value
+            if rc == abi.HD_E_CANCELLED:
+                return False
+            check(rc, self.h)
+        self.t = t.value
+        return True
 
     def halfstep_vz(self, t):
         check(lib().hd_halfstep_vz(self.h, t), self.h)

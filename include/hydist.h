@@ -171,6 +171,24 @@ int hd_step(hd_ctx *ctx, double *t);
  * entry and the final time on return. */
 int hd_run(hd_ctx *ctx, double *t, double Tmax);
 
+/* Hydraulic_Calculation's loop with output: as hd_run, and after every
+ * step at which t is a whole multiple of `interval` seconds
+ * (engine.cpp:145,243) the stream is synchronised and cb(ctx, t, user) is
+ * called (it may copy fields out); a nonzero return stops the run with
+ * HD_E_CANCELLED. */
+typedef int (*hd_output_cb)(hd_ctx *ctx, double t, void *user);
+int hd_run_cb(hd_ctx *ctx, double *t, double Tmax, int interval, hd_output_cb cb, void *user);
+
+/* Stage entry points for per-stage use and parity.  hd_sediment_j: the
+ * first half-step's FS block, Find_VTH + hesoK, Scan_FSj, the solve,
+ * FSj_extract_solution and Update_FS (engine.cpp:149-165);
+ * hd_sediment_i: the second half-step's (engine.cpp:258-282);
+ * hd_bedload: Calculate_Qb + BedLoad (sediment_transport.cpp:352-537,
+ * intended form).  t is the half-step's time. */
+int hd_sediment_j(hd_ctx *ctx, double t);
+int hd_sediment_i(hd_ctx *ctx, double t);
+int hd_bedload(hd_ctx *ctx, double t);
+
 /* Block until all work queued on the context has finished. */
 int hd_sync(hd_ctx *ctx);
 

@@ -1668,6 +1668,35 @@ int ho_halfstep_vz(ho_ctx *ctx, double t) {
     return HD_OK;
 }
 
+int ho_sediment_j(ho_ctx *ctx, double t) {
+    if (!ctx) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    find_vth(o);
+    heso_k(o);
+    fs_sweep_j(o, t, 3, o.M - 1);
+    update_fs(o);
+    return HD_OK;
+}
+
+int ho_sediment_i(ho_ctx *ctx, double t) {
+    if (!ctx) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    find_vth(o);
+    heso_k(o);
+    fs_sweep_i(o, t, 3, o.N - 1);
+    update_fs(o);
+    return HD_OK;
+}
+
+int ho_bedload(ho_ctx *ctx, double t) {
+    (void)t;
+    if (!ctx) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    calculate_qb(o);
+    bed_load(o);
+    return HD_OK;
+}
+
 int ho_step(ho_ctx *ctx, double *t) {
     if (!ctx || !t) return HD_E_INVALID;
     if (!ctx->o.inited) return fail(&ctx->o, HD_E_STATE, "init first");

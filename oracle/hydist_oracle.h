@@ -35,6 +35,11 @@ int ho_init(ho_ctx *o);
 int ho_update_boundary(ho_ctx *o, double t);
 int ho_halfstep_uz(ho_ctx *o, double t);
 int ho_halfstep_vz(ho_ctx *o, double t);
+/* Stage blocks (engine.cpp:149-165, :258-282; sediment_transport.cpp
+ * Calculate_Qb + BedLoad), as hd_sediment_j / hd_sediment_i / hd_bedload. */
+int ho_sediment_j(ho_ctx *ctx, double t);
+int ho_sediment_i(ho_ctx *ctx, double t);
+int ho_bedload(ho_ctx *ctx, double t);
 int ho_step(ho_ctx *o, double *t);
 int ho_run(ho_ctx *o, double *t, double Tmax);
 int ho_copy_out(ho_ctx *o, int field, double *dst, size_t n);

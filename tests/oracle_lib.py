@@ -35,7 +35,8 @@ def lib():
                                 C.POINTER(abi.Config), C.POINTER(vp)]
         for name in ["ho_init", "ho_destroy"]:
             getattr(L, name).argtypes = [vp]
-        for name in ["ho_update_boundary", "ho_halfstep_uz", "ho_halfstep_vz"]:
+        for name in ["ho_update_boundary", "ho_halfstep_uz", "ho_halfstep_vz", "ho_sediment_j", "ho_sediment_i",
+                     "ho_bedload"]:
             getattr(L, name).argtypes = [vp, C.c_double]
         L.ho_step.argtypes = [vp, dp]
         L.ho_run.argtypes = [vp, dp, C.c_double]
@@ -112,6 +113,10 @@ class Oracle:
 
     def halfstep_uz(self, t):
         assert lib().ho_halfstep_uz(self.h, t) == 0
+
+    def stage(self, name, t):
+        """Stage block: 'sediment_j', 'sediment_i' or 'bedload'."""
+        assert getattr(lib(), "ho_" + name)(self.h, t) == 0
 
     def halfstep_vz(self, t):
         assert lib().ho_halfstep_vz(self.h, t) == 0

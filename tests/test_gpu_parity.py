@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
-from hydist_c_amd import scenarios as S
+from hydist_c_amd import engine, scenarios as S
 from hydist_c_amd.abi import HD_QUIRK_Q11_LITERAL
 from parity_util import HYDRO, bits_equal, diff_report, make_pair, ulp_diff
 
@@ -160,3 +160,38 @@ def test_narrow_channel_modes_bitwise(mode):
     o.step(15)
     rep = diff_report(g, o, HYDRO)
     assert all(v["bitwise"] for v in rep.values()), rep
+
+
+def test_stage_entry_points_match_oracle():
+    """hd_sediment_j / hd_sediment_i / hd_bedload against the oracle's
+    stage blocks on the same mid-run state (FS and dH involve pow/log, so a
+    relative tolerance; the hydraulic fields are untouched)."""
+    _print_case("stages")
+    sc = S.open_box(40, 28, seed=3)
+    sc.sediment_start = sc.bed_change_start = 1e9  # stages run only when called
+    g, o = make_pair(sc, sediment=True, bed_change=True)
+    g.step(8)
+    o.step(8)
+    for name, t in (("sediment_j", 17.0), ("sediment_i", 18.0), ("bedload", 18.0)):
+        g.stage(name, t)
+        o.stage(name, t)
+    for f in ("FS", "DH", "QBX", "QBY", "VTH"):
+        a, b = g.get(f), o.get(f)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-18), (f, float(np.max(np.abs(a - b))))
+    assert float(np.abs(g.get("FS")).max()) > 0
+    rep = diff_report(g, o, HYDRO)
+    assert all(v["bitwise"] for v in rep.values()), rep
+
+
+def test_run_with_output_callback():
+    _print_case("run_cb")
+    sc = S.open_box(40, 28, seed=3)
+    g = engine.Simulation(sc)
+    g.init()
+    seen = []
+    assert g.run(20.0, interval=4, callback=lambda t: seen.append(t) or False)
+    assert seen == [4.0, 8.0, 12.0, 16.0, 20.0] and g.t == 20.0
+    stopped = []
+    assert not g.run(40.0, interval=2, callback=lambda t: stopped.append(t) or t >= 26.0)
+    assert stopped[-1] == 26.0 and g.t == 26.0
+    g.close()
