@@ -457,3 +457,27 @@ def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0
     s = meander_islands(M, N, seed=k, n_islands=n_isl, Q=-4000.0 * N / 1024, hours=int(np.ceil(T / 3600.0)),
                         sediment_start=0.0, name=f"request{k}_{M}x{N}")
     return s, T, form, sed
+
+
+def straight_channel(M=160, N=8, depth=4.0, manning=0.03, Q=-10.0, z_down=0.0, hours=4, visc=5.0,
+                     name=None):
+    """Flat-bed straight channel along j, wet on rows 3..N-1 (rows 2 and N
+    land, so the side lines carry no boundary): discharge Q enters on the up
+    side (j = M), the down side (j = 2) holds z_down.  The steady state has
+    a water surface whose slope balances bed friction."""
+    def in_channel(y, x):
+        return (y > 1.0) & (y < N - 2.0)
+
+    def dep(y, x):
+        return np.full(np.broadcast(y, x).shape, depth)
+
+    h = build_bathymetry(M, N, in_channel, dep)
+    shape = h.shape
+    total_time = hours + 3
+    zeros = _series(0.0, total_time)
+    return Scenario(
+        name=name or f"straight_{M}x{N}", M=M, N=N, h=h, hsnham=np.full(shape, manning),
+        visc=np.full(shape, visc), Fw=np.zeros(shape), total_time=total_time, boundary_type=(2, 0, 0, 0),
+        bc=[_series(Q, total_time), _series(z_down, total_time), zeros, zeros], CC=None,
+        meta={"Q": Q, "manning": manning, "depth": depth},
+    )

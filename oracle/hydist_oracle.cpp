@@ -22,11 +22,17 @@
  * source_chs/source_nchs), and whole-trajectory invariants (lake at rest
  * stays exactly 0, closed-basin volume, inflow balance, uniform FS stays
  * uniform, assembled systems are solved).
- * Parity unpinned: the friction / advection / viscosity coefficients of
- * the momentum rows (_calculate_abcd, uSolver, vSolver), the smoothing
- * weights (_normalize), Find_VTH / hesoK, and the bed-change terms
- * (Calculate_Qb, _bed_load; never launched by the reference, restated in
- * their intended form) have no check beyond those invariants.
+ * tests/test_oracle_kernels.py adds closed forms for single kernels:
+ * smoothing (constant and linear fields, a hand case with a dry
+ * neighbour), Find_VTH, hesoK (scaling, averaging, clamps), Calculate_Qb
+ * (direction, threshold, exponent 2.1), BedLoad's divergence term, and a
+ * steady channel in which surface slope balances the friction term
+ * (Ky1 = g hbar^(-2 mu_mn) n^2) and continuity holds.
+ * Parity unpinned: the advection and viscosity coefficients of the
+ * momentum rows (zero in every case above), BedLoad's one-sided
+ * differences near dry cells and its FS-diffusion term, and the
+ * non-cohesive source (source_nchs) have no check beyond the whole-run
+ * invariants.
  *
  * Parallelism: OpenMP over independent lines (sweeps) and rows (stencils).
  * Every parallel loop writes disjoint data, so results do not depend on
@@ -1694,6 +1700,21 @@ int ho_bedload(ho_ctx *ctx, double t) {
     Oracle &o = ctx->o;
     calculate_qb(o);
     bed_load(o);
+    return HD_OK;
+}
+
+int ho_run_stage(ho_ctx *ctx, const char *name) {
+    if (!ctx || !name) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    std::vector<double> tmp;
+    const std::string n = name;
+    if (n == "normalize_u") normalize(o, true, tmp);
+    else if (n == "normalize_v") normalize(o, false, tmp);
+    else if (n == "find_vth") find_vth(o);
+    else if (n == "heso_k") heso_k(o);
+    else if (n == "calculate_qb") calculate_qb(o);
+    else if (n == "bed_load") bed_load(o);
+    else return HD_E_INVALID;
     return HD_OK;
 }
 

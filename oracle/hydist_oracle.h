@@ -40,6 +40,10 @@ int ho_halfstep_vz(ho_ctx *o, double t);
 int ho_sediment_j(ho_ctx *ctx, double t);
 int ho_sediment_i(ho_ctx *ctx, double t);
 int ho_bedload(ho_ctx *ctx, double t);
+/* One kernel restatement on the current state, for closed-form checks:
+ * "normalize_u", "normalize_v", "find_vth", "heso_k", "calculate_qb",
+ * "bed_load". */
+int ho_run_stage(ho_ctx *ctx, const char *name);
 int ho_step(ho_ctx *o, double *t);
 int ho_run(ho_ctx *o, double *t, double Tmax);
 int ho_copy_out(ho_ctx *o, int field, double *dst, size_t n);

@@ -39,6 +39,7 @@ def lib():
                      "ho_bedload"]:
             getattr(L, name).argtypes = [vp, C.c_double]
         L.ho_step.argtypes = [vp, dp]
+        L.ho_run_stage.argtypes = [vp, C.c_char_p]
         L.ho_run.argtypes = [vp, dp, C.c_double]
         for name in ["ho_copy_out", "ho_copy_in", "ho_copy_out_boundary", "ho_copy_in_boundary"]:
             getattr(L, name).argtypes = [vp, C.c_int, dp, C.c_size_t]
@@ -113,6 +114,10 @@ class Oracle:
 
     def halfstep_uz(self, t):
         assert lib().ho_halfstep_uz(self.h, t) == 0
+
+    def kernel(self, name):
+        """One kernel restatement on the current state (ho_run_stage)."""
+        assert lib().ho_run_stage(self.h, name.encode()) == 0, name
 
     def stage(self, name, t):
         """Stage block: 'sediment_j', 'sediment_i' or 'bedload'."""
