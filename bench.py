@@ -50,11 +50,12 @@ def stage_bytes(name, sim, sc):
     # line sweeps over (L-1) positions of every line
     lines_uz, lines_vz = sc.M - 1, sc.N - 1
     pos_uz, pos_vz = sc.N - 1, sc.M - 1
-    if name.startswith(("k_cr", "k_thomas", "k_hyd_assemble", "k_hyd_extract", "k_fs_assemble", "k_fs_extract")):
+    if name.startswith(("k_cr", "k_part", "k_thomas", "k_hyd_assemble", "k_hyd_extract", "k_fs_assemble",
+                        "k_fs_extract")):
         uz = any(t in name for t in ("uz", "fs_j", "<j>"))
         n = (lines_uz * pos_uz) if uz else (lines_vz * pos_vz)
         hyd = "fs" not in name
-        if name.startswith(("k_cr", "k_thomas")):
+        if name.startswith(("k_cr", "k_part", "k_thomas")):
             # tridiagonal solve: 4 coefficients in + solution out per unknown
             return (80.0 if hyd else 40.0) * n
         if name.startswith("k_hyd_assemble"):
@@ -103,23 +104,36 @@ def measured_traffic(symbol):
     return None
 
 
-def kernel_symbol(stage, sc=None):
-    """The kernel (as rocprofv3 names it) behind a stage name."""
-    if stage.startswith("k_cr_reg<") and sc is not None:
-        prec, what = stage[9:-1].split(",")
-        t = "double, false" if prec == "f64" else "float, true"
-        # rows per thread: ceil(n / threads) rounded up to 2, 4 or 8
-        n = (sc.N - 1 if what in ("uz", "fs_j") else sc.M - 1)
-        threads = 512 if n > 1024 else 256
-        kpt = -(-n // threads)
-        k = 2 if kpt <= 2 else 4 if kpt <= 4 else 8
-        return f"k_cr_reg<{t}, {'false' if what.startswith('fs') else 'true'}, {k}>"
-    if stage.startswith("k_cr<"):
-        prec, what = stage[5:-1].split(",")
-        t = "double, false" if prec == "f64" else "float, true"
-        return f"k_cr<{t}, {'false' if what.startswith('fs') else 'true'}>"
+SWEEP_ORDER = ("uz", "vz", "fs_j", "fs_i")  # order of the autotuner's "chosen" list
+
+
+def kernel_symbol(stage, sc=None, chosen=None, layout=None):
+    """The kernel (as rocprofv3 names it) behind a stage name; `chosen` is
+    the autotuner's choice per sweep (e.g. ["part/4", "cr/256", ...]),
+    `layout` the sweep buffers' layout (0 plain form, 1 the fast forms)."""
+    for prefix, kind in (("k_part<", "part"), ("k_cr_reg<", "cr")):
+        if stage.startswith(prefix) and sc is not None:
+            prec, what = stage[len(prefix):-1].split(",")
+            t = "double, false" if prec == "f64" else "float, true"
+            il = "false" if what.startswith("fs") else "true"
+            n = sc.N - 1 if what in ("uz", "fs_j") else sc.M - 1  # unknowns per line
+            pick = chosen[SWEEP_ORDER.index(what)] if chosen else None
+            param = int(pick.split("/")[1]) if pick and pick.startswith(kind + "/") else 0
+            if kind == "part":
+                k = param or (4 if n <= 2048 else 8)
+            else:
+                threads = param or (512 if n > 1024 else 256)
+                kpt = -(-n // threads)
+                k = 2 if kpt <= 2 else 4 if kpt <= 4 else 8
+            return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}>"
     if stage.startswith("k_thomas"):
         return "k_thomas"
+    # assembly / extraction: template <direction (1 = UZ / FS along j), layout
+    # (0 = [rho][line] for the plain form, 1 = [line][rho])>
+    for prefix in ("k_hyd_assemble<", "k_hyd_extract<", "k_fs_assemble<", "k_fs_extract<"):
+        if stage.startswith(prefix) and layout is not None:
+            what = stage[len(prefix):-1]
+            return f"{prefix}{1 if what in ('uz', 'j') else 0}, {layout}>"
     return stage
 
 
@@ -130,7 +144,7 @@ def time_form(sc, form, steps, warmup, profile_steps):
     # advance to the sediment switch-on, then the warm-up steps
     while sim.t < sc.sediment_start:
         sim.step()
-    # solver workgroup sizes (results do not depend on them)
+    # line solver per sweep (partition K or cyclic-reduction workgroup size)
     tune = engine.autotune(sim) if form != 0 else None
     sim.step(warmup)
     sim.sync()
@@ -255,8 +269,8 @@ def main():
                     help="switch sediment and bed change on at t=0 (short profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has a single MI355X, and this build "
-              "has no multi-GPU path (see DESIGN.md for how the mesh would shard).", file=sys.stderr)
+        print("bench.py measures one GPU only: this library has no multi-GPU path "
+              "(the mesh would shard by row bands with halo exchange; not built).", file=sys.stderr)
         sys.exit(2)
 
     from hydist_c_amd import abi, scenarios as S
@@ -287,7 +301,8 @@ def main():
     for name, (launches, total_ms) in stats.items():
         if launches == 0 or total_ms <= 0:
             continue
-        g = groups.setdefault(kernel_symbol(name, sc), [0, 0.0, 0.0, True])
+        chosen = r["tune"]["chosen"] if r["tune"] else None
+        g = groups.setdefault(kernel_symbol(name, sc, chosen, 0 if best == "plain" else 1), [0, 0.0, 0.0, True])
         b = stage_bytes(name, sim, sc)
         g[0] += launches
         g[1] += total_ms
@@ -311,7 +326,7 @@ def main():
     # the tridiagonal solve kernel of the best form, at its own roofline
     tri = None
     solver = [(k, v) for k, v in stats.items()
-              if k.startswith(("k_cr", "k_thomas")) and "fs" not in k and v[0] > 0 and v[1] > 0]
+              if k.startswith(("k_cr", "k_part", "k_thomas")) and "fs" not in k and v[0] > 0 and v[1] > 0]
     if solver:
         k, (launches, total_ms) = max(solver, key=lambda kv: kv[1][1])
         b = stage_bytes(k, sim, sc)
@@ -358,7 +373,7 @@ def main():
         "config": {"workload": "2048x1024 meander+islands, sediment + FP64 bed change, dt=2s",
                    "M": sc.M, "N": sc.N, "form": best, "timed_from_t": sc.sediment_start},
         "forms": {k: {"cell_steps_per_s": v["cell_steps_per_s"], "ms_per_step": v["ms_per_step"],
-                      "finite": v["finite"], "solver_threads": v["tune"]["chosen"] if v["tune"] else None,
+                      "finite": v["finite"], "solver": v["tune"]["chosen"] if v["tune"] else None,
                       "profile_top": sorted(({"stage": a, "launches": b[0], "avg_ms": b[1] / max(b[0], 1)}
                                              for a, b in v["stats"].items()), key=lambda d: -d["avg_ms"])[:6]}
                   for k, v in results.items()},

@@ -991,18 +991,57 @@ int hd_autotune(hd_ctx *ctx, char *report, size_t n) {
     std::string r = "{";
     int rc = hd::tune_line_solvers(ctx, ctx->ds, ctx->hs, ctx->nscen, r);
     if (rc) return rc;
-    char buf[128];
-    std::snprintf(buf, sizeof(buf), "%s\"chosen\": [%d, %d, %d, %d]}", r.size() > 1 ? ", " : "",
-                  ctx->cr_threads[1][1], ctx->cr_threads[0][1], ctx->cr_threads[1][0], ctx->cr_threads[0][0]);
-    r += buf;
+    // choices in the order uz, vz, fs_j, fs_i
+    r += r.size() > 1 ? ", \"chosen\": [" : "\"chosen\": [";
+    const int order[4][2] = {{1, 1}, {0, 1}, {1, 0}, {0, 0}};
+    for (int k = 0; k < 4; k++) {
+        const int kind = ctx->solver_kind[order[k][0]][order[k][1]];
+        const int param = ctx->solver_param[order[k][0]][order[k][1]];
+        char buf[48];
+        std::snprintf(buf, sizeof(buf), "%s\"%s/%d\"", k ? ", " : "",
+                      kind == HD_SOLVER_CR ? "cr" : (kind == HD_SOLVER_PARTITION ? "part" : "default"), param);
+        r += buf;
+    }
+    r += "]}";
     if (report && n) std::snprintf(report, n, "%s", r.c_str());
     return HD_OK;
 }
 
+int hd_set_solver(hd_ctx *ctx, int dir, int hydraulic, int kind, int param) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (dir != 0 && dir != 1) return set_err(ctx, HD_E_INVALID, "dir must be 0 (VZ) or 1 (UZ)");
+    bool ok = false;
+    if (kind == HD_SOLVER_DEFAULT) ok = param == 0;
+    if (kind == HD_SOLVER_PARTITION) ok = param == 0 || param == 2 || param == 4 || param == 8;
+    if (kind == HD_SOLVER_CR) ok = param >= 0 && param <= 512 && param % 64 == 0;
+    if (!ok)
+        return set_err(ctx, HD_E_INVALID,
+                       "solver: DEFAULT with 0, PARTITION with K in {0, 2, 4, 8}, or CR with a workgroup size "
+                       "that is a multiple of 64 up to 512 (0 = default)");
+    ctx->solver_kind[dir][hydraulic ? 1 : 0] = kind;
+    ctx->solver_param[dir][hydraulic ? 1 : 0] = param;
+    return HD_OK;
+}
+
 int hd_set_solver_threads(hd_ctx *ctx, int dir, int hydraulic, int threads) {
-    if (!ctx || (dir != 0 && dir != 1) || threads < 0 || threads > 512 || threads % 64)
-        return set_err(ctx, HD_E_INVALID, "dir must be 0/1 and threads a multiple of 64 up to 512 (0 = default)");
-    ctx->cr_threads[dir][hydraulic ? 1 : 0] = threads;
+    return hd_set_solver(ctx, dir, hydraulic, HD_SOLVER_CR, threads);
+}
+
+int hd_gtsv_batch(const double *dl, const double *d, const double *du, const double *b, double *x, int m, int nsys,
+                  int ld, int precision, void *stream) {
+    if (!dl || !d || !du || !b || !x) return set_err(nullptr, HD_E_INVALID, "hd_gtsv_batch: NULL buffer");
+    if (m < 1 || m > 4096 || nsys < 0 || (nsys > 1 && ld < m) || (precision != 0 && precision != 1))
+        return set_err(nullptr, HD_E_INVALID, "hd_gtsv_batch: need 1 <= m <= 4096, nsys >= 0, ld >= m, precision 0/1");
+    if (nsys == 0) return HD_OK;
+    hd_ctx c;  // launch settings and error text only; owns nothing
+    c.stream = (hipStream_t)stream;
+    c.cfg.form = precision ? HD_FORM_FP32_REFINE : HD_FORM_FP64_FAST;
+    int rc = hd::solve_batch(&c, dl, d, du, b, x, m, nsys, nsys > 1 ? ld : m);
+    if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) {
+        c.err = std::string("hd_gtsv_batch: ") + hipGetErrorString(hipGetLastError());
+        rc = HD_E_HIP;
+    }
+    if (rc) return set_err(nullptr, rc, c.err);
     return HD_OK;
 }
 

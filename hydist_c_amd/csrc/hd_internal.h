@@ -23,12 +23,16 @@ struct hd_ctx {
     int nscen = 1;
     hd::DevState *ds = nullptr;    // descriptors of all scenarios (device copy)
     double *clock = nullptr;       // device time of the current half-step
-    // whole-step graphs, keyed by which optional stages the step runs
-    // cyclic-reduction workgroup size per sweep [dir][hydraulic], 0 = the
-    // default rule; set by hd_autotune (results do not depend on it)
-    int cr_threads[2][2] = {{0, 0}, {0, 0}};
+    // line solver of the fast forms per sweep [dir][hydraulic]: the kind
+    // (HD_SOLVER_*, HD_SOLVER_DEFAULT = the default rule) and its parameter
+    // (unknowns per thread for the partition solver, workgroup size for
+    // cyclic reduction; 0 = that solver's default); set by hd_autotune or
+    // hd_set_solver
+    int solver_kind[2][2] = {{0, 0}, {0, 0}};
+    int solver_param[2][2] = {{0, 0}, {0, 0}};
     bool use_graph = false;
     bool capturing = false;
+    // whole-step graphs, keyed by which optional stages the step runs
     std::map<int, hipGraphExec_t> graphs;
     hd::SweepBufs sweep{};         // line-sweep buffers
     std::vector<void *> allocs;

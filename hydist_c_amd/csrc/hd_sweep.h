@@ -35,6 +35,11 @@ struct SweepBufs {
 
 int sweep_rows(const DevState &s, int dir, bool hyd);
 int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report);
+// nsys independent systems of m rows, ld elements apart, solved into x by
+// the partition solver on ctx->stream (FP64, or FP32 + refinement when
+// ctx->cfg.form is HD_FORM_FP32_REFINE).
+int solve_batch(hd_ctx *ctx, const double *dl, const double *d, const double *du, const double *b, double *x, int m,
+                int nsys, int ld);
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,
                    int end, double t, double s_start);
 

@@ -537,108 +537,39 @@ HD_DEV void cr_replay(CRShared<T> sh, int n) {
     }
 }
 
-// z-row q of an interleaved line (rows 2q-1, 2q, 2q+1) reduced in FP64.
-HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__ Bv,
-                        const double *__restrict__ C, const double *__restrict__ D, int q, int nrows) {
-    const int r = 2 * q;
-    Row o{0.0, Bv[r], 0.0, D[r]};
-    const double ar = A[r], cr = C[r];
-    if (r - 1 >= 0 && ar != 0.0) {
-        const double al = ar / Bv[r - 1];
-        o.a = -al * A[r - 1];
-        o.b = o.b - al * C[r - 1];
-        o.d = o.d - al * D[r - 1];
+// A z row of an interleaved line with its neighbouring velocity rows
+// eliminated in FP64: zr is row 2q, m1 row 2q-1 (used when has_m1), p1 row
+// 2q+1 (used when has_p1).
+HD_DEV Row reduce_z_vals(const Row &m1, const Row &zr, const Row &p1, bool has_m1, bool has_p1) {
+    Row o{0.0, zr.b, 0.0, zr.d};
+    if (has_m1 && zr.a != 0.0) {
+        const double al = zr.a / m1.b;
+        o.a = -al * m1.a;
+        o.b = o.b - al * m1.c;
+        o.d = o.d - al * m1.d;
     }
-    if (r + 1 < nrows && cr != 0.0) {
-        const double ga = cr / Bv[r + 1];
-        o.c = -ga * C[r + 1];
-        o.b = o.b - ga * A[r + 1];
-        o.d = o.d - ga * D[r + 1];
+    if (has_p1 && zr.c != 0.0) {
+        const double ga = zr.c / p1.b;
+        o.c = -ga * p1.c;
+        o.b = o.b - ga * p1.a;
+        o.d = o.d - ga * p1.d;
     }
     return o;
 }
 
-// The FP64 reduced rows (for the residual) are kept in the Ap/Bp scratch.
-template <typename T, bool REFINE, bool INTERLEAVED>
-__global__ void __launch_bounds__(512) k_cr(const DevState *__restrict__ S, SweepBufs Bin, int start, int end,
-                                           int nrows) {
-    extern __shared__ unsigned char smem[];
-    const SweepBufs B = Bin.scen(blockIdx.y);
-    const int line = start + blockIdx.x;
-    if (line > end) return;
-    const int ld = B.ld;
-    const double *__restrict__ A = B.A + (size_t)line * ld;
-    const double *__restrict__ Bv = B.B + (size_t)line * ld;
-    const double *__restrict__ C = B.C + (size_t)line * ld;
-    const double *__restrict__ D = B.D + (size_t)line * ld;
-    double *__restrict__ X = B.X + (size_t)line * ld;
-    // FP64 reduced system kept for the residual (global scratch Ap/Bp: a,b | c,d)
-    double *__restrict__ Ra = B.Ap + (size_t)line * ld;
-    double *__restrict__ Rb = Ra + ld / 2;
-    double *__restrict__ Rc = B.Bp + (size_t)line * ld;
-    double *__restrict__ Rd = Rc + ld / 2;
-    const int n = INTERLEAVED ? nrows / 2 : nrows;
-    T *base = reinterpret_cast<T *>(smem);
-    CRShared<T> sh{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n, base + 5 * n};
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int q = tid; q < n; q += nt) {
-        Row r;
-        if (INTERLEAVED)
-            r = reduce_z_row(A, Bv, C, D, q, nrows);
-        else
-            r = Row{A[q], Bv[q], C[q], D[q]};
-        sh.a[q] = (T)r.a;
-        sh.b[q] = (T)r.b;
-        sh.c[q] = (T)r.c;
-        sh.d[q] = (T)r.d;
-        if (REFINE) {
-            Ra[q] = r.a;
-            Rb[q] = r.b;
-            Rc[q] = r.c;
-            Rd[q] = r.d;
-        }
-    }
-    __syncthreads();
-    cr_solve<T>(sh, n, REFINE);
-    if (REFINE) {
-        // z0 -> X (FP64), residual of the FP64 system -> d
-        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
-        __syncthreads();
-        for (int q = tid; q < n; q += nt) {
-            const int stp = INTERLEAVED ? 2 : 1;
-            const double xq = X[stp * q];
-            double rr = Rd[q] - Rb[q] * xq;
-            if (q > 0) rr = rr - Ra[q] * X[stp * (q - 1)];
-            if (q + 1 < n) rr = rr - Rc[q] * X[stp * (q + 1)];
-            sh.d[q] = (T)rr;
-        }
-        __syncthreads();
-        cr_replay<T>(sh, n);
-        for (int q = tid; q < n; q += nt) {
-            const int stp = INTERLEAVED ? 2 : 1;
-            X[stp * q] = X[stp * q] + (double)sh.d[q];
-        }
-    } else {
-        for (int q = tid; q < n; q += nt) X[INTERLEAVED ? 2 * q : q] = (double)sh.d[q];
-    }
-    if (INTERLEAVED) {
-        __syncthreads();
-        // velocity rows from their own FP64 rows
-        for (int q = tid; q < n; q += nt) {
-            const int r = 2 * q + 1;
-            if (r >= nrows) continue;
-            double v = D[r];
-            const double a = A[r], c = C[r];
-            if (a != 0.0) v = v - a * X[r - 1];
-            if (c != 0.0 && r + 1 < nrows) v = v - c * X[r + 1];
-            X[r] = v / Bv[r];
-        }
-    }
+// z-row q of an interleaved line (rows 2q-1, 2q, 2q+1) reduced in FP64.
+HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__ Bv,
+                        const double *__restrict__ C, const double *__restrict__ D, int q, int nrows) {
+    const int r = 2 * q;
+    const bool has_m1 = r - 1 >= 0, has_p1 = r + 1 < nrows;
+    const Row zr{A[r], Bv[r], C[r], D[r]};
+    const Row m1 = has_m1 ? Row{A[r - 1], Bv[r - 1], C[r - 1], D[r - 1]} : identity_row();
+    const Row p1 = has_p1 ? Row{A[r + 1], Bv[r + 1], C[r + 1], D[r + 1]} : identity_row();
+    return reduce_z_vals(m1, zr, p1, has_m1, has_p1);
 }
 
-// Register-resident variant (not enabled: see kRegisterCR).
-// One workgroup per line; thread t owns rows q = t + k * blockDim.x,
-// k < KPT.  The FP64 rows a thread needs again (its velocity row for the
+// Cyclic reduction of a whole line in LDS, one workgroup per line; thread
+// t owns rows q = t + k * blockDim.x, k < KPT.  The FP64 rows a thread needs again (its velocity row for the
 // recovery, its reduced z row for the residual) stay in its registers, so
 // each coefficient is read from HBM once and each unknown written once.
 template <typename T, bool REFINE, bool INTERLEAVED, int KPT>
@@ -752,6 +683,233 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 }
 
 // ---------------------------------------------------------------------------
+// Partition solver, one workgroup per line ([line][rho]).
+//
+// Thread t owns the K consecutive unknowns q = tK .. tK+K-1 of the line's
+// system (for hydraulics the z-only system, its rows reduced in FP64 as in
+// reduce_z_vals).  It eliminates its first K-1 unknowns in registers
+// (Thomas), leaving each as
+//     x_k = y_k + v_k * L_{t-1} + w_k * L_t,
+// where L_t is the thread's last unknown.  Substituted into the last row,
+// this gives one row per thread in the L: a tridiagonal system (a Schur
+// complement of the line's, so diagonally dominant like it) solved by
+// cyclic reduction in LDS, after which every x_k follows.  Rows pass
+// through LDS on the way in and out so that global accesses stay coalesced
+// while each thread works on a contiguous chunk.  With REFINE the T
+// solution is corrected once by solving for the FP64 residual with the
+// same elimination (den, cp, v, w and the reduced system's multipliers).
+// Needs blockDim.x * K >= the number of unknowns.
+// ---------------------------------------------------------------------------
+template <typename T, bool REFINE, bool INTERLEAVED, int K>
+__global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows) {
+    static_assert(K >= 2, "a chunk needs at least one interior unknown");
+    constexpr int R = INTERLEAVED ? 2 * K : K;  // rows per thread
+    const SweepBufs B = Bin.scen(blockIdx.y);
+    extern __shared__ double smem_part[];
+    const int line = start + blockIdx.x;
+    if (line > end) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int n = INTERLEAVED ? nrows / 2 : nrows;
+    const size_t lo = (size_t)line * B.ld;
+    // LDS.  The staging buffer keeps row rho at rho + rho / R, so that the R
+    // rows of one thread are read with few bank conflicts.  The exchange
+    // arrays and the reduced system overlap it: they are used only between
+    // the inbound and the outbound staging, with barriers on both sides.
+    double *stage = smem_part;
+    T *ey = reinterpret_cast<T *>(smem_part);           // y_0 of each thread
+    T *ev = reinterpret_cast<T *>(smem_part + nt);      // v_0
+    T *ew = reinterpret_cast<T *>(smem_part + 2 * nt);  // w_0
+    double *ez = smem_part + 3 * nt;                    // x_0 in FP64
+    T *crb = reinterpret_cast<T *>(smem_part + 4 * nt);
+    CRShared<T> sh{crb, crb + nt, crb + 2 * nt, crb + 3 * nt, crb + 4 * nt, crb + 5 * nt};
+
+    // Inbound: array a is read coalesced (element tid + m * nt) into
+    // registers while array a-1 goes through LDS; rows past nrows read as
+    // identity rows.
+    double own[4][R], prv[4];
+    double cur[R], nxt[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+        const int e = tid + m * nt;
+        cur[m] = e < nrows ? B.A[lo + e] : 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        if (a < 3) {
+            const double *nsrc = a == 0 ? B.B : (a == 1 ? B.C : B.D);
+            const double fill = a == 0 ? 1.0 : 0.0;
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                const int e = tid + m * nt;
+                nxt[m] = e < nrows ? nsrc[lo + e] : fill;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < R; m++) {
+            const int e = tid + m * nt;
+            stage[e + e / R] = cur[m];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int e = tid * R + r;
+            own[a][r] = stage[e + e / R];
+        }
+        if (INTERLEAVED) {
+            const int e = tid * R - 1;
+            prv[a] = tid > 0 ? stage[e + e / R] : (a == 1 ? 1.0 : 0.0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < R; m++) cur[m] = nxt[m];
+    }
+
+    // The thread's K rows (FP64) and, for hydraulics, its velocity rows.
+    Row red[K], vel[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int q = tid * K + k;
+        if (INTERLEAVED) {
+            const Row m1 = k == 0 ? Row{prv[0], prv[1], prv[2], prv[3]}
+                                  : Row{own[0][2 * k - 1], own[1][2 * k - 1], own[2][2 * k - 1], own[3][2 * k - 1]};
+            const Row zr{own[0][2 * k], own[1][2 * k], own[2][2 * k], own[3][2 * k]};
+            const Row p1{own[0][2 * k + 1], own[1][2 * k + 1], own[2][2 * k + 1], own[3][2 * k + 1]};
+            // rows 2q + 1 < nrows exactly when q < n (nrows is even)
+            red[k] = q < n ? reduce_z_vals(m1, zr, p1, q > 0, true) : identity_row();
+            vel[k] = p1;
+        } else {
+            Row r{own[0][k], own[1][k], own[2][k], own[3][k]};
+            if (q == 0) r.a = 0.0;
+            if (q + 1 >= n) r.c = 0.0;
+            red[k] = q < n ? r : identity_row();
+        }
+    }
+
+    // Local elimination of the interior unknowns 0..K-2.
+    T la[K], lc[K], den[K], cp[K], yv[K], vv[K], wv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        la[k] = (T)red[k].a;
+        lc[k] = (T)red[k].c;
+    }
+    den[0] = (T)red[0].b;
+    cp[0] = lc[0] / den[0];
+    yv[0] = (T)red[0].d / den[0];
+    vv[0] = -la[0] / den[0];
+#pragma unroll
+    for (int k = 1; k < K - 1; k++) {
+        den[k] = (T)red[k].b - la[k] * cp[k - 1];
+        cp[k] = lc[k] / den[k];
+        yv[k] = ((T)red[k].d - la[k] * yv[k - 1]) / den[k];
+        vv[k] = -(la[k] * vv[k - 1]) / den[k];
+    }
+    wv[K - 2] = -cp[K - 2];
+#pragma unroll
+    for (int k = K - 3; k >= 0; k--) {
+        yv[k] = yv[k] - cp[k] * yv[k + 1];
+        vv[k] = vv[k] - cp[k] * vv[k + 1];
+        wv[k] = -(cp[k] * wv[k + 1]);
+    }
+
+    // Row of L_t: a x_{K-2} + b L_t + c x'_0 = d, with x'_0 the next
+    // thread's first unknown (zero past the last thread).
+    ey[tid] = yv[0];
+    ev[tid] = vv[0];
+    ew[tid] = wv[0];
+    __syncthreads();
+    {
+        const bool nb = tid + 1 < nt;
+        const T yn = nb ? ey[tid + 1] : T(0), vn = nb ? ev[tid + 1] : T(0), wn = nb ? ew[tid + 1] : T(0);
+        const T aK = la[K - 1], cK = lc[K - 1];
+        sh.a[tid] = aK * vv[K - 2];
+        sh.b[tid] = (T)red[K - 1].b + aK * wv[K - 2] + cK * vn;
+        sh.c[tid] = cK * wn;
+        sh.d[tid] = (T)red[K - 1].d - aK * yv[K - 2] - cK * yn;
+    }
+    __syncthreads();
+    cr_solve<T>(sh, nt, REFINE);
+
+    double zf[K];
+    {
+        const T Lt = sh.d[tid], Lm = tid > 0 ? sh.d[tid - 1] : T(0);
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) zf[k] = (double)(yv[k] + vv[k] * Lm + wv[k] * Lt);
+        zf[K - 1] = (double)Lt;
+    }
+    if (REFINE) {
+        // FP64 residual of the thread's rows at the T solution ...
+        ez[tid] = zf[0];
+        __syncthreads();
+        const double zn = tid + 1 < nt ? ez[tid + 1] : 0.0;
+        const double zm = tid > 0 ? (double)sh.d[tid - 1] : 0.0;
+        T rv[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int q = tid * K + k;
+            const double xl = k == 0 ? zm : zf[k - 1];
+            const double xr = k == K - 1 ? zn : zf[k + 1];
+            double v = red[k].d - red[k].b * zf[k];
+            if (q > 0) v = v - red[k].a * xl;
+            if (q + 1 < n) v = v - red[k].c * xr;
+            rv[k] = (T)v;
+        }
+        // ... solved with the same elimination (v_k, w_k do not change).
+        // ey and sh.d are free again: every read of them happened before
+        // one of the barriers above.
+        T yr[K];
+        yr[0] = rv[0] / den[0];
+#pragma unroll
+        for (int k = 1; k < K - 1; k++) yr[k] = (rv[k] - la[k] * yr[k - 1]) / den[k];
+#pragma unroll
+        for (int k = K - 3; k >= 0; k--) yr[k] = yr[k] - cp[k] * yr[k + 1];
+        ey[tid] = yr[0];
+        __syncthreads();
+        const T yn = tid + 1 < nt ? ey[tid + 1] : T(0);
+        sh.d[tid] = rv[K - 1] - la[K - 1] * yr[K - 2] - lc[K - 1] * yn;
+        __syncthreads();
+        cr_replay<T>(sh, nt);
+        const T Ct = sh.d[tid], Cm = tid > 0 ? sh.d[tid - 1] : T(0);
+#pragma unroll
+        for (int k = 0; k < K - 1; k++) zf[k] = zf[k] + (double)(yr[k] + vv[k] * Cm + wv[k] * Ct);
+        zf[K - 1] = zf[K - 1] + (double)Ct;
+    }
+
+    double out[R];
+    if (INTERLEAVED) {
+        // velocities from their own FP64 rows; the z after the chunk is the
+        // next thread's first (ez was last read before a barrier above)
+        ez[tid] = zf[0];
+        __syncthreads();
+        const double zn = tid + 1 < nt ? ez[tid + 1] : 0.0;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int r = 2 * (tid * K + k) + 1;
+            out[2 * k] = zf[k];
+            double v = vel[k].d;
+            if (vel[k].a != 0.0) v = v - vel[k].a * zf[k];
+            const double zr = k == K - 1 ? zn : zf[k + 1];
+            if (vel[k].c != 0.0 && r + 1 < nrows) v = v - vel[k].c * zr;
+            out[2 * k + 1] = v / vel[k].b;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] = zf[k];
+    }
+    __syncthreads();  // the exchange and reduced-system arrays are dead from here
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int e = tid * R + r;
+        stage[e + e / R] = out[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+        const int e = tid + m * nt;
+        if (e < nrows) B.X[lo + e] = stage[e + e / R];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Extraction (hydraulics).  Define: cells before `last` are written before
 // the end cell, whose continuity formula uses the new velocity at last-1.
 // ---------------------------------------------------------------------------
@@ -857,11 +1015,6 @@ int sweep_rows(const DevState &s, int dir, bool hyd) {
 
 // Lets a kernel use more than the default 64 KiB of dynamic LDS (gfx950
 // has 160 KiB per CU).
-// The register-resident solver (k_cr_reg) is switched off: the first GPU
-// run that contained it (with batched contexts and whole-step graphs) ended
-// in a device fault whose cause has not been found.
-constexpr bool kRegisterCR = true;
-
 template <typename K>
 static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
     if (bytes <= 65536) return HD_OK;
@@ -871,6 +1024,98 @@ static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
         return HD_E_HIP;
     }
     return HD_OK;
+}
+
+// LDS of k_part: the staging buffer, or the exchange arrays plus the
+// reduced system, whichever is larger.
+template <typename T, bool REFINE>
+static size_t part_lds_bytes(int nt, int rows_per_thread) {
+    const size_t stage = (size_t)nt * (rows_per_thread + 1) * sizeof(double);
+    const size_t other = (size_t)nt * 4 * sizeof(double) + (size_t)nt * (REFINE ? 6 : 4) * sizeof(T);
+    return stage > other ? stage : other;
+}
+
+// Launch shape of a line solver for lines of nz unknowns: workgroup size
+// and unknowns per thread (the template K).  Returns false, with the reason
+// in ctx->err, when the line is too long for that solver.
+static bool solver_shape(hd_ctx *ctx, int nz, int kind, int param, int &threads, int &kpt) {
+    if (kind == HD_SOLVER_CR) {
+        threads = param ? param : (nz > 1024 ? 512 : 256);
+        const int need = (nz + threads - 1) / threads;
+        if (need > 8) {
+            ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds cyclic reduction's 8 x " +
+                       std::to_string(threads);
+            return false;
+        }
+        kpt = need <= 2 ? 2 : (need <= 4 ? 4 : 8);
+        return true;
+    }
+    kpt = param ? param : (nz <= 4 * 512 ? 4 : 8);
+    const int chunks = (nz + kpt - 1) / kpt;
+    threads = std::max(64, (chunks + 63) / 64 * 64);
+    if (threads > 512) {
+        ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the partition solver's 512 x " +
+                   std::to_string(kpt);
+        return false;
+    }
+    return true;
+}
+
+// Launches one fast-form line solver over `lines` lines of nscen scenarios.
+template <typename T, bool REFINE, bool IL>
+static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int kind, int threads,
+                         int kpt, int start, int end, int nrows, const char *name) {
+    hipStream_t st = ctx->stream;
+    const int lines = end - start + 1;
+    int rc = HD_OK;
+    if (kind == HD_SOLVER_CR) {
+        const int nz = IL ? nrows / 2 : nrows;
+        const size_t sm = (REFINE ? 6 : 4) * (size_t)nz * sizeof(T);
+        auto go = [&](auto kern) {
+            rc = allow_lds(ctx, kern, sm);
+            if (rc) return;
+            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(S, B, start, end, nrows)));
+        };
+        if (kpt == 2) go(k_cr_reg<T, REFINE, IL, 2>);
+        else if (kpt == 4) go(k_cr_reg<T, REFINE, IL, 4>);
+        else go(k_cr_reg<T, REFINE, IL, 8>);
+    } else {
+        const size_t sm = part_lds_bytes<T, REFINE>(threads, IL ? 2 * kpt : kpt);
+        auto go = [&](auto kern) {
+            rc = allow_lds(ctx, kern, sm);
+            if (rc) return;
+            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows)));
+        };
+        if (kpt == 2) go(k_part<T, REFINE, IL, 2>);
+        else if (kpt == 4) go(k_part<T, REFINE, IL, 4>);
+        else go(k_part<T, REFINE, IL, 8>);
+    }
+    return rc;
+}
+
+// Stage names of the fast-form solvers, [kind == CR][f32][dir][hydraulic].
+static const char *solver_name(int kind, bool f32, int dir, bool hyd) {
+    static const char *names[2][2][2][2] = {
+        {{{"k_part<f64,fs_i>", "k_part<f64,vz>"}, {"k_part<f64,fs_j>", "k_part<f64,uz>"}},
+         {{"k_part<f32r,fs_i>", "k_part<f32r,vz>"}, {"k_part<f32r,fs_j>", "k_part<f32r,uz>"}}},
+        {{{"k_cr_reg<f64,fs_i>", "k_cr_reg<f64,vz>"}, {"k_cr_reg<f64,fs_j>", "k_cr_reg<f64,uz>"}},
+         {{"k_cr_reg<f32r,fs_i>", "k_cr_reg<f32r,vz>"}, {"k_cr_reg<f32r,fs_j>", "k_cr_reg<f32r,uz>"}}}};
+    return names[kind == HD_SOLVER_CR][f32][dir][hyd];
+}
+
+static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int dir, bool hyd, int kind,
+                      int param, int start, int end, int nrows) {
+    const int nz = hyd ? nrows / 2 : nrows;
+    if (kind == HD_SOLVER_DEFAULT) kind = HD_SOLVER_PARTITION;
+    int threads, kpt;
+    if (!solver_shape(ctx, nz, kind, param, threads, kpt)) return HD_E_UNSUPPORTED;
+    const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
+    const char *name = solver_name(kind, f32, dir, hyd);
+    if (f32)
+        return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name)
+                   : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name);
+    return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name)
+               : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name);
 }
 
 template <int DIR, int LAYOUT>
@@ -888,69 +1133,14 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
                k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, ctx->capturing ? 1 : 0));
     const int lines = end - start + 1;
-    const int form = ctx->cfg.form;
     if (LAYOUT == ROW_MAJOR) {
         LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
     } else {
-        if (!kRegisterCR) {
-            const int nz = hyd ? nrows / 2 : nrows;
-            const int tuned = ctx->cr_threads[DIR][hyd ? 1 : 0];
-            const int threads = tuned ? tuned : (nz > 1024 ? 512 : 256);
-            if (form == HD_FORM_FP32_REFINE) {
-                const size_t sm = 6 * (size_t)nz * sizeof(float);
-                int rc = allow_lds(ctx, hyd ? k_cr<float, true, true> : k_cr<float, true, false>, sm);
-                if (rc) return rc;
-                if (hyd)
-                    LAUNCH(ctx, DIR ? "k_cr<f32r,uz>" : "k_cr<f32r,vz>",
-                           (k_cr<float, true, true><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-                else
-                    LAUNCH(ctx, DIR ? "k_cr<f32r,fs_j>" : "k_cr<f32r,fs_i>",
-                           (k_cr<float, true, false><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-            } else {
-                const size_t sm = 4 * (size_t)nz * sizeof(double);
-                int rc = allow_lds(ctx, hyd ? k_cr<double, false, true> : k_cr<double, false, false>, sm);
-                if (rc) return rc;
-                if (hyd)
-                    LAUNCH(ctx, DIR ? "k_cr<f64,uz>" : "k_cr<f64,vz>",
-                           (k_cr<double, false, true><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-                else
-                    LAUNCH(ctx, DIR ? "k_cr<f64,fs_j>" : "k_cr<f64,fs_i>",
-                           (k_cr<double, false, false><<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-            }
-        } else {
-            const int nz = hyd ? nrows / 2 : nrows;
-            const int tuned = ctx->cr_threads[DIR][hyd ? 1 : 0];
-            int threads = nz > 1024 ? 512 : 256;
-            if (tuned && (nz + tuned - 1) / tuned <= 8) threads = tuned;
-            const int kpt = (nz + threads - 1) / threads;
-            if (kpt > 8) {
-                ctx->err = "line of " + std::to_string(nz) + " unknowns exceeds the solver's 4096";
-                return HD_E_UNSUPPORTED;
-            }
-            const bool f32 = form == HD_FORM_FP32_REFINE;
-            const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
-            const char *name = f32 ? (hyd ? (DIR ? "k_cr_reg<f32r,uz>" : "k_cr_reg<f32r,vz>") : (DIR ? "k_cr_reg<f32r,fs_j>" : "k_cr_reg<f32r,fs_i>"))
-                                   : (hyd ? (DIR ? "k_cr_reg<f64,uz>" : "k_cr_reg<f64,vz>") : (DIR ? "k_cr_reg<f64,fs_j>" : "k_cr_reg<f64,fs_i>"));
-            int rc = HD_OK;
-            auto go = [&](auto kern) {
-                rc = allow_lds(ctx, kern, sm);
-                if (rc) return;
-                LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, n), threads, sm, st>>>(S, B, start, end, nrows)));
-            };
-            auto pick = [&](auto kp) {
-                constexpr int K = decltype(kp)::value;
-                if (f32) {
-                    if (hyd) go(k_cr_reg<float, true, true, K>); else go(k_cr_reg<float, true, false, K>);
-                } else {
-                    if (hyd) go(k_cr_reg<double, false, true, K>); else go(k_cr_reg<double, false, false, K>);
-                }
-            };
-            if (kpt <= 2) pick(std::integral_constant<int, 2>{});
-            else if (kpt <= 4) pick(std::integral_constant<int, 4>{});
-            else pick(std::integral_constant<int, 8>{});
-            if (rc) return rc;
-        }
+        const int h = hyd ? 1 : 0;
+        int rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
+                            nrows);
+        if (rc) return rc;
     }
     const Launch2D g = grid2d(s, n);
     if (hyd)
@@ -962,10 +1152,9 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     return HD_OK;
 }
 
-// Times the cyclic-reduction solve of one sweep for each workgroup size
-// and keeps the fastest.  The system is assembled once from the current
-// state; only sweep scratch is written, and every workgroup size gives the
-// same result (each row's update is independent of which lane does it).
+// Times every line-solver candidate on one sweep and keeps the fastest.
+// The system is assembled once from the current state; only sweep scratch
+// is written, so the simulation state is untouched.
 template <int DIR>
 static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
                      double t, double s_start, std::string &report) {
@@ -979,92 +1168,71 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     else
         LAUNCH(ctx, "k_fs_assemble",
                (k_fs_assemble<DIR, LINE_MAJOR><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, 0)));
-    const int lines = end - start + 1, nz = hyd ? nrows / 2 : nrows;
-    const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
-    const size_t sm = f32 ? 6 * (size_t)nz * sizeof(float) : 4 * (size_t)nz * sizeof(double);
+    const int nz = hyd ? nrows / 2 : nrows;
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return HD_E_HIP;
-    int best = 0;
+    if (hipEventCreate(&e0) != hipSuccess) return HD_E_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return HD_E_HIP;
+    }
+    const bool prof = ctx->prof;
+    ctx->prof = false;  // candidates are timed here, not as stages
+    int best_kind = 0, best_param = 0, rc = HD_OK;
     float best_ms = 0;
     char buf[160];
-    for (int threads : {128, 256, 512}) {
-        int rc = HD_OK;
-        const int kpt = (nz + threads - 1) / threads;
-        if (kRegisterCR && kpt > 8) continue;
-        auto reg = [&](auto kp) {
-            constexpr int K = decltype(kp)::value;
-            if (f32) {
-                if (hyd) k_cr_reg<float, true, true, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-                else k_cr_reg<float, true, false, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-            } else {
-                if (hyd) k_cr_reg<double, false, true, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-                else k_cr_reg<double, false, false, K><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-            }
-        };
-        auto run = [&]() {
-            if (kRegisterCR) {
-                if (kpt <= 2) reg(std::integral_constant<int, 2>{});
-                else if (kpt <= 4) reg(std::integral_constant<int, 4>{});
-                else reg(std::integral_constant<int, 8>{});
-            } else if (f32) {
-                if (hyd) k_cr<float, true, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-                else k_cr<float, true, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-            } else {
-                if (hyd) k_cr<double, false, true><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-                else k_cr<double, false, false><<<lines, threads, sm, st>>>(S, B, start, end, nrows);
-            }
-            rc = check_launch(ctx, "cyclic reduction (tuning)");
-        };
-        if (sm > 65536) {
-            // the LDS attribute of every variant this tuner may launch
-            for (const void *k : {(const void *)k_cr<float, true, true>, (const void *)k_cr<float, true, false>,
-                                  (const void *)k_cr<double, false, true>, (const void *)k_cr<double, false, false>,
-                                  (const void *)k_cr_reg<float, true, true, 8>, (const void *)k_cr_reg<float, true, false, 8>,
-                                  (const void *)k_cr_reg<double, false, true, 8>, (const void *)k_cr_reg<double, false, false, 8>,
-                                  (const void *)k_cr_reg<float, true, true, 4>, (const void *)k_cr_reg<float, true, false, 4>,
-                                  (const void *)k_cr_reg<double, false, true, 4>, (const void *)k_cr_reg<double, false, false, 4>,
-                                  (const void *)k_cr_reg<float, true, true, 2>, (const void *)k_cr_reg<float, true, false, 2>,
-                                  (const void *)k_cr_reg<double, false, true, 2>, (const void *)k_cr_reg<double, false, false, 2>})
-                if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm) != hipSuccess) {
-                    rc = HD_E_HIP;
-                    break;
-                }
-            if (rc) break;
-        }
-        run();  // warm-up
-        (void)hipEventRecord(e0, st);
-        for (int r = 0; r < 3 && !rc; r++) run();
-        (void)hipEventRecord(e1, st);
-        if (rc || hipEventSynchronize(e1) != hipSuccess) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return rc ? rc : HD_E_HIP;
-        }
+    const int cands[6][2] = {{HD_SOLVER_PARTITION, 2}, {HD_SOLVER_PARTITION, 4}, {HD_SOLVER_PARTITION, 8},
+                             {HD_SOLVER_CR, 128},      {HD_SOLVER_CR, 256},      {HD_SOLVER_CR, 512}};
+    for (const auto &c : cands) {
+        int threads, kpt;
+        if (!solver_shape(ctx, nz, c[0], c[1], threads, kpt)) continue;  // too long for this candidate
+        rc = run_solver(ctx, S, B, n, DIR, hyd, c[0], c[1], start, end, nrows);  // warm-up
+        if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? HD_OK : HD_E_HIP;
+        for (int r = 0; r < 3 && !rc; r++) rc = run_solver(ctx, S, B, n, DIR, hyd, c[0], c[1], start, end, nrows);
+        if (!rc) rc = hipEventRecord(e1, st) == hipSuccess ? HD_OK : HD_E_HIP;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? HD_OK : HD_E_HIP;
+        if (rc) break;
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= 3;
-        std::snprintf(buf, sizeof(buf), "%s\"%s%s/%d\": %.4f", report.size() > 1 ? ", " : "", DIR ? "uz" : "vz",
-                      hyd ? "" : "_fs", threads, ms);
+        std::snprintf(buf, sizeof(buf), "%s\"%s%s/%s%d\": %.4f", report.size() > 1 ? ", " : "", DIR ? "uz" : "vz",
+                      hyd ? "" : "_fs", c[0] == HD_SOLVER_CR ? "cr" : "part", c[1], ms);
         report += buf;
-        if (!best || ms < best_ms) {
-            best = threads;
+        if (!best_kind || ms < best_ms) {
+            best_kind = c[0];
+            best_param = c[1];
             best_ms = ms;
         }
     }
+    ctx->prof = prof;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ctx->cr_threads[DIR][hyd ? 1 : 0] = best;
+    if (rc) return rc;
+    if (!best_kind) return HD_E_UNSUPPORTED;  // ctx->err names the line length
+    ctx->solver_kind[DIR][hyd ? 1 : 0] = best_kind;
+    ctx->solver_param[DIR][hyd ? 1 : 0] = best_param;
     return HD_OK;
 }
 
 int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report) {
-    if (ctx->cfg.form == HD_FORM_FP64_PLAIN || n != 1) return HD_OK;  // Thomas: nothing to tune
+    if (ctx->cfg.form == HD_FORM_FP64_PLAIN) return HD_OK;  // Thomas: nothing to tune
     const hd_options &o = ctx->ops;
     int rc = tune_impl<1>(ctx, S, s, n, true, 2, s.M, 0.0, 0.0, report);
     if (!rc) rc = tune_impl<0>(ctx, S, s, n, true, 2, s.N, 0.0, 0.0, report);
     if (!rc && ctx->cfg.sediment) rc = tune_impl<1>(ctx, S, s, n, false, 3, s.M - 1, o.sediment_start, o.bed_change_start, report);
     if (!rc && ctx->cfg.sediment) rc = tune_impl<0>(ctx, S, s, n, false, 3, s.N - 1, o.sediment_start, o.bed_change_start, report);
     return rc;
+}
+
+int solve_batch(hd_ctx *ctx, const double *dl, const double *d, const double *du, const double *b, double *x, int m,
+                int nsys, int ld) {
+    SweepBufs B{};
+    B.A = const_cast<double *>(dl);
+    B.B = const_cast<double *>(d);
+    B.C = const_cast<double *>(du);
+    B.D = const_cast<double *>(b);
+    B.X = x;
+    B.ld = ld;
+    return run_solver(ctx, nullptr, B, 1, 0, false, HD_SOLVER_PARTITION, 0, 0, nsys - 1, m);
 }
 
 int run_line_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir, bool hyd, int start,

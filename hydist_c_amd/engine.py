@@ -60,6 +60,8 @@ def lib():
         L.hd_set_graph.argtypes = [vp, C.c_int]
         L.hd_autotune.argtypes = [vp, C.c_char_p, C.c_size_t]
         L.hd_set_solver_threads.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.hd_set_solver.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.hd_gtsv_batch.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         L.hd_snapshot_size.argtypes = [vp, C.POINTER(C.c_size_t)]
         L.hd_snapshot.argtypes = [vp, C.c_void_p, C.c_size_t, C.c_double]
         L.hd_restore.argtypes = [vp, C.c_void_p, C.c_size_t, dp]
@@ -230,7 +232,9 @@ class Simulation:
 
 
 def autotune(sim):
-    """Picks the fastest solver workgroup size per sweep; returns the report."""
+    """Times the line-solver candidates of every sweep and keeps the fastest
+    per sweep; returns the report ({candidate: ms, "chosen": [uz, vz, fs_j,
+    fs_i]} with choices such as "part/4" or "cr/256")."""
     import json
     buf = C.create_string_buffer(2048)
     check(lib().hd_autotune(sim.h, buf, 2048), sim.h)
@@ -238,7 +242,23 @@ def autotune(sim):
 
 
 def set_solver_threads(sim, direction, hydraulic, threads):
+    """Cyclic reduction with this workgroup size (0 = its default) for one sweep."""
     check(lib().hd_set_solver_threads(sim.h, direction, int(hydraulic), threads), sim.h)
+
+
+SOLVER_DEFAULT, SOLVER_PARTITION, SOLVER_CR = 0, 1, 2
+
+
+def set_solver(sim, direction, hydraulic, kind, param=0):
+    """Line solver of one sweep: SOLVER_PARTITION with K unknowns per thread
+    (2, 4, 8; 0 = default), SOLVER_CR with a workgroup size, or
+    SOLVER_DEFAULT."""
+    check(lib().hd_set_solver(sim.h, direction, int(hydraulic), kind, param), sim.h)
+
+
+def gtsv_batch(dl, d, du, b, x, m, nsys, ld, precision=0, stream=None):
+    """hd_gtsv_batch on device pointers (ints); raises on error."""
+    check(lib().hd_gtsv_batch(dl, d, du, b, x, m, nsys, ld, precision, stream), None)
 
 
 def _stage_api():

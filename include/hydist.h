@@ -265,16 +265,45 @@ int hd_batch_copy_in(hd_ctx *ctx, int scen, int field, const double *src, size_t
  * time it is needed.  Results are identical to eager launches. */
 int hd_set_graph(hd_ctx *ctx, int on);
 
-/* Autotuning (no counterpart in the reference).  Times the fast forms'
- * cyclic-reduction solve of each sweep (UZ, VZ, and the FS sweeps when
- * sediment is on) at several workgroup sizes on the context's current
- * state and keeps the fastest; writes a JSON report (ms per candidate and
- * the choice) into report.  Results do not depend on the choice, so tuning
- * never changes a run's output.  hd_set_solver_threads sets one choice by
- * hand (dir 1 = UZ, 0 = VZ; threads a multiple of 64 up to 512, 0 = the
- * default rule). */
+/* Line solvers of the fast forms (they replace the Thomas solve,
+ * UVZSolver_multithread.cpp:19-43, and the unused SPIKE solver,
+ * gtsv.cpp:88-193).  One workgroup solves one line:
+ *   HD_SOLVER_PARTITION: each thread eliminates K consecutive unknowns in
+ *     registers, the one interface unknown per thread is solved by cyclic
+ *     reduction in LDS (param = K: 2, 4 or 8; lines up to 512*K unknowns);
+ *   HD_SOLVER_CR: cyclic reduction of the whole line in LDS (param = the
+ *     workgroup size, a multiple of 64 up to 512; lines up to 8 x that).
+ * HD_SOLVER_DEFAULT is the partition solver with K = 4 (K = 8 above 2048
+ * unknowns).  The two solvers, and the partition solver at different K,
+ * round differently (agreement ~1e-15 relative in FP64); the CR workgroup
+ * size does not change any bit. */
+enum { HD_SOLVER_DEFAULT = 0, HD_SOLVER_PARTITION = 1, HD_SOLVER_CR = 2 };
+
+/* Autotuning (no counterpart in the reference).  Times each line solver
+ * candidate (partition at K = 2, 4, 8; cyclic reduction at 128, 256, 512
+ * threads) on every sweep (UZ, VZ, and the FS sweeps when sediment is on)
+ * from the context's current state, keeps the fastest per sweep, and
+ * writes a JSON report (ms per candidate and the choices) into report.
+ * The choice can change results at rounding level (see above); it is
+ * deterministic once made.  hd_set_solver sets one choice by hand
+ * (dir 1 = UZ, 0 = VZ; hydraulic 1 = the z/u sweep, 0 = the FS sweep);
+ * hd_set_solver_threads(ctx, dir, hyd, t) is hd_set_solver(ctx, dir, hyd,
+ * HD_SOLVER_CR, t). */
 int hd_autotune(hd_ctx *ctx, char *report, size_t n);
+int hd_set_solver(hd_ctx *ctx, int dir, int hydraulic, int kind, int param);
 int hd_set_solver_threads(hd_ctx *ctx, int dir, int hydraulic, int threads);
+
+/* Batched tridiagonal solve on device buffers, the drop-in for
+ * gtsv_spike_partial_diag_pivot_v1 (gtsv.cpp:88, declared gtsv.h:7; same
+ * argument meaning: dl/d/du the sub-, main and super-diagonals, b the
+ * right-hand side, x the solution, m rows; dl[0] and du[m-1] are ignored)
+ * extended to nsys systems stored ld elements apart.  Solved by the
+ * partition solver without pivoting, so the systems must be diagonally
+ * dominant (every system this library forms is); precision 0 = FP64,
+ * 1 = FP32 with one FP64 refinement.  Synchronous on stream (a
+ * hipStream_t, NULL = the null stream).  m <= 4096. */
+int hd_gtsv_batch(const double *dl, const double *d, const double *du, const double *b, double *x,
+                  int m, int nsys, int ld, int precision, void *stream);
 
 /* Checkpoint / restart.  Replaces the reference's stubs (the -c/-T
  * options, main.cpp:283-284, loader.cpp:172-185): the snapshot holds every

@@ -228,6 +228,13 @@ hipError_t hipFree(void *p) {
     g_allocs.erase(it);
     return hipSuccess;
 }
+// "Device" buffers for tests that call entry points taking device pointers.
+extern "C" void *hd_emu_alloc(size_t n) {
+    void *p = nullptr;
+    return hipMalloc(&p, n) == hipSuccess ? p : nullptr;
+}
+extern "C" void hd_emu_free(void *p) { (void)hipFree(p); }
+
 static void no_capture(const char *what) {
     if (g_capturing) {
         std::fprintf(stderr, "[emu] %s during graph capture\n", what);

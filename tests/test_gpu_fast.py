@@ -1,7 +1,8 @@
 """GPU fast forms against the CPU oracle.
 
-HD_FORM_FP64_FAST solves the z-only (Schur) system of each line by cyclic
-reduction in FP64; HD_FORM_FP32_REFINE solves it in FP32 and adds one FP64
+HD_FORM_FP64_FAST solves the z-only (Schur) system of each line in FP64 by
+the partition solver (or, when chosen, cyclic reduction);
+HD_FORM_FP32_REFINE solves it in FP32 and adds one FP64
 residual-refinement solve.  Both differ from the reference's interleaved
 Thomas only by round-off, so they are compared with tolerances:
 
@@ -12,7 +13,7 @@ Thomas only by round-off, so they are compared with tolerances:
 import numpy as np
 import pytest
 
-from hydist_c_amd import scenarios as S
+from hydist_c_amd import engine, scenarios as S
 from hydist_c_amd.abi import HD_FORM_FP32_REFINE, HD_FORM_FP64_FAST
 from parity_util import make_pair
 
@@ -43,24 +44,28 @@ def test_fast_form_close_to_oracle(form, maker, steps):
 
 @pytest.mark.parametrize("form", [HD_FORM_FP64_FAST, HD_FORM_FP32_REFINE], ids=NAMES.get)
 def test_fast_form_wetting_drying(form):
-    """Tidal flats (cells dry and re-wet): the upwind switches on
-    H_moi <= H_TINH (UVZSolver_multithread.cpp:186-201, 252-268) flip
-    under round-off near the front, so differences grow beyond round-off.
-    Bar: |dZ| <= 1e-4 m, |dU|, |dV| <= 1e-4 m/s after 120 steps; mask
-    differences are reported and must stay below 1% of the wet cells."""
+    """Tidal flats: cells dry (33 of them change state in the first 100
+    steps) and the upwind switches on H_moi <= H_TINH
+    (UVZSolver_multithread.cpp:186-201, 252-268) act at the front.
+    Bar after 100 steps: identical masks, |dZ|, |dU|, |dV| <= 1e-6.
+    The run stops at 100 steps on purpose: from about step 110 the front
+    of this case produces velocity spikes in the FP64 oracle itself (|U|
+    of several m/s on films of a few cm), where differences of round-off
+    size grow tenfold every few steps for every solver, the oracle's own
+    Thomas included against any reordering of it."""
     sc = S.tidal_flats(64, 40)
     print(f"[case] fast-flats {NAMES[form]}", flush=True)
     g, o = make_pair(sc, form=form)
-    g.step(120)
-    o.step(120)
+    k0 = o.get("KHOUOT").copy()
+    g.step(100)
+    o.step(100)
     kg, ko = g.get("KHOUOT"), o.get("KHOUOT")
-    nd = int((kg != ko).sum())
-    print(f"[flats] {NAMES[form]} mask differences: {nd}", flush=True)
-    assert nd <= 0.01 * (ko == 0).sum()
+    assert int((ko != k0).sum()) >= 20, "the case no longer dries cells"
+    assert np.array_equal(kg, ko)
     for f in ("Z", "U", "V"):
         d = np.max(np.abs(g.get(f) - o.get(f)))
         print(f"[flats] {NAMES[form]} max|d{f}| = {d:.3e}", flush=True)
-        assert d <= 1e-4, (f, d)
+        assert d <= 1e-6, (f, d)
 
 
 @pytest.mark.parametrize("form", [HD_FORM_FP64_FAST, HD_FORM_FP32_REFINE], ids=NAMES.get)
@@ -95,3 +100,30 @@ def test_fast_form_one_hour_config1(form):
     print(f"[1h] {NAMES[form]} dZ={dz:.3e} dU={du:.3e} dV={dv:.3e}", flush=True)
     assert dz <= 1e-3 and du <= 1e-4 and dv <= 1e-4
     assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+
+
+SOLVERS = [(engine.SOLVER_PARTITION, 2), (engine.SOLVER_PARTITION, 4), (engine.SOLVER_PARTITION, 8),
+           (engine.SOLVER_CR, 0)]
+
+
+@pytest.mark.parametrize("form", [HD_FORM_FP64_FAST, HD_FORM_FP32_REFINE], ids=NAMES.get)
+@pytest.mark.parametrize("kind,param", SOLVERS, ids=["part2", "part4", "part8", "cr"])
+def test_every_line_solver_close_to_oracle(form, kind, param):
+    """Each line solver, on every sweep (hydraulic and FS), against the
+    oracle at the fast forms' tolerance."""
+    sc = S.meander_islands(160, 96, n_islands=8)
+    sc.sediment_start = sc.bed_change_start = 4.0
+    print(f"[case] solver {NAMES[form]} kind={kind} K/threads={param}", flush=True)
+    g, o = make_pair(sc, form=form, sediment=True, bed_change=True)
+    for d in (0, 1):
+        for hyd in (0, 1):
+            engine.set_solver(g, d, hyd, kind, param)
+    g.step(10)
+    o.step(10)
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for f in ("Z", "U", "V"):
+        d = np.max(np.abs(g.get(f) - o.get(f)))
+        assert d <= TOL[form], (f, d)
+    a, b = g.get("FS"), o.get("FS")
+    scale = np.abs(b).max() + 1e-300
+    assert np.max(np.abs(a - b)) <= 1e3 * TOL[form] * scale
