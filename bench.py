@@ -130,6 +130,11 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
         return "k_thomas"
     # assembly / extraction: template <direction (1 = UZ / FS along j), layout
     # (0 = [rho][line] for the plain form, 1 = [line][rho])>
+    # (column lines into [line][rho] are assembled by the LDS-tiled kernels)
+    if layout == 1 and stage == "k_hyd_assemble<uz>":
+        return "k_hyd_assemble_cols<1>"
+    if layout == 1 and stage == "k_fs_assemble<j>":
+        return "k_fs_assemble_cols<1>"
     for prefix in ("k_hyd_assemble<", "k_hyd_extract<", "k_fs_assemble<", "k_fs_extract<"):
         if stage.startswith(prefix) and layout is not None:
             what = stage[len(prefix):-1]

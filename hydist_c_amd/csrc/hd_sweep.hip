@@ -271,6 +271,133 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Column lines (UZ and FS along j) into [line][rho]: the state is read with
+// threads along j, so each line's rows leave through an LDS transpose.
+// A tile covers 64 lines and TR = 8 * CPT cells along i, starting at
+// i0 = 2 + TR * blockIdx.y so that each line's run of rows starts on a
+// 64 * CPT-byte (FS) or 128 * CPT-byte (hydraulics) boundary; each thread
+// assembles CPT cells (i0 + threadIdx.y + 8c).  Two of the four coefficient
+// arrays pass through LDS at a time, which halves the LDS of a block
+// against staging all four.  Same rows as k_hyd_assemble / k_fs_assemble
+// with DIR = 1.  One cell per thread measured fastest on the 2048 x 1024
+// mesh (against 2 and 4: more blocks in flight beat longer write runs).
+// ---------------------------------------------------------------------------
+constexpr int COLS_HYD_CPT = 1;
+constexpr int COLS_FS_CPT = 1;
+
+template <int CPT>
+__global__ void __launch_bounds__(512) k_hyd_assemble_cols(const DevState *__restrict__ S, SweepBufs Bin,
+                                                           int start, int end) {
+    constexpr int TR = 8 * CPT;
+    const SweepBufs B = Bin.scen(blockIdx.z);
+    const DevState &s = S[blockIdx.z];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int j = blockIdx.x * 64 + tx;
+    const int i0 = 2 + blockIdx.y * TR;
+    const int L = s.N;
+    const double f4 = 2 * s.c.g * s.c.dTchia2dX;
+    Row rz[CPT], ru[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int i = i0 + ty + 8 * c;
+        rz[c] = identity_row();
+        ru[c] = identity_row();
+        int k, f, l;
+        if (j >= start && j <= end && i <= L && locate<1>(s, j, i, k, f, l)) {
+            const SegCtx sc = seg_ctx<1>(s, j, f, l);
+            hyd_rows<1>(s, j, i, sc, f4, rz[c], ru[c]);
+            if (i == f) {
+                B.sys0[(size_t)j * s.capV + k] = sc.rho0;
+                B.sysn[(size_t)j * s.capV + k] = sc.g.sn;
+            }
+        }
+    }
+    __shared__ double t0[TR][2 * 64 + 1], t1[TR][2 * 64 + 1];
+    const int t = ty * 64 + tx, ld = B.ld;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int cy = ty + 8 * c;
+            t0[cy][2 * tx] = pass ? rz[c].c : rz[c].a;
+            t0[cy][2 * tx + 1] = pass ? ru[c].c : ru[c].a;
+            t1[cy][2 * tx] = pass ? rz[c].d : rz[c].b;
+            t1[cy][2 * tx + 1] = pass ? ru[c].d : ru[c].b;
+        }
+        __syncthreads();
+        double *o0 = pass ? B.C : B.A, *o1 = pass ? B.D : B.B;
+        for (int e = t; e < 64 * 2 * TR; e += 512) {
+            const int ln = e / (2 * TR), r = e % (2 * TR);  // r runs fastest: contiguous in [line][rho]
+            const int jj = blockIdx.x * 64 + ln, ci = i0 + r / 2;
+            if (jj < start || jj > end || ci > L) continue;
+            const size_t o = lm_idx(2 * (ci - 2) + (r & 1), jj, ld);
+            o0[o] = t0[r / 2][2 * ln + (r & 1)];
+            o1[o] = t1[r / 2][2 * ln + (r & 1)];
+        }
+        if (pass == 0) __syncthreads();
+    }
+}
+
+template <int CPT>
+__global__ void __launch_bounds__(512) k_fs_assemble_cols(const DevState *__restrict__ S, SweepBufs Bin, int start,
+                                                          int end, double t_in, double s_start, int from_clock) {
+    constexpr int TR = 8 * CPT;
+    const SweepBufs B = Bin.scen(blockIdx.z);
+    const DevState &s = S[blockIdx.z];
+    const double tm = from_clock ? s.clock[0] : t_in;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int j = blockIdx.x * 64 + tx;
+    const int i0 = 2 + blockIdx.y * TR;
+    const int L = s.N;
+    Row rw[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const int i = i0 + ty + 8 * c;
+        rw[c] = identity_row();
+        if (j < start || j > end || i > L) continue;
+        int k, f, l;
+        if (locate<1>(s, j, i, k, f, l) && l - f >= 2) {
+            bool ran1, ran2;
+            wall_flags(s, 1, j, f, l, ran1, ran2);
+            if (i >= f + 1 && i <= l - 1) {
+                fs_row_j(s, i, j, f, l, ran1, ran2, tm, s_start, rw[c].a, rw[c].b, rw[c].c, rw[c].d);
+                if (i == f + 1) rw[c].a = 0.0;
+                if (i == l - 1) rw[c].c = 0.0;
+            }
+            if (i == f) {
+                B.sys0[(size_t)j * s.capV + k] = f + 1 - 2;
+                B.sysn[(size_t)j * s.capV + k] = l - f - 2;
+            }
+        } else if (k >= 0 && i == f) {
+            B.sys0[(size_t)j * s.capV + k] = -1;
+            B.sysn[(size_t)j * s.capV + k] = -1;
+        }
+    }
+    __shared__ double t0[TR][64 + 1], t1[TR][64 + 1];
+    const int t = ty * 64 + tx, ld = B.ld;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int c = 0; c < CPT; c++) {
+            const int cy = ty + 8 * c;
+            t0[cy][tx] = pass ? rw[c].c : rw[c].a;
+            t1[cy][tx] = pass ? rw[c].d : rw[c].b;
+        }
+        __syncthreads();
+        double *o0 = pass ? B.C : B.A, *o1 = pass ? B.D : B.B;
+        for (int e = t; e < 64 * TR; e += 512) {
+            const int ln = e / TR, r = e % TR;
+            const int jj = blockIdx.x * 64 + ln, ci = i0 + r;
+            if (jj < start || jj > end || ci > L) continue;
+            const size_t o = lm_idx(ci - 2, jj, ld);
+            o0[o] = t0[r][ln];
+            o1[o] = t1[r][ln];
+        }
+        if (pass == 0) __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Thomas, one lane per line, rows of all lines in lockstep ([rho][line]).
 // Rows are read in blocks of TB with the next block's loads issued before
 // the current block's serial recurrence, so memory latency overlaps it.
@@ -1013,6 +1140,35 @@ int sweep_rows(const DevState &s, int dir, bool hyd) {
     return hyd ? 2 * (L - 1) : (L - 1);
 }
 
+// Assembly of one sweep's rows (column lines in [line][rho] take the
+// LDS-transposing tile kernels).
+template <int DIR, int LAYOUT>
+static int launch_assembly(hd_ctx *ctx, const DevState *S, const DevState &s, const SweepBufs &B, int n, bool hyd,
+                           int start, int end, double t, double s_start, int from_clock) {
+    hipStream_t st = ctx->stream;
+    if (DIR == 1 && LAYOUT == LINE_MAJOR) {
+        constexpr int TRH = 8 * COLS_HYD_CPT, TRF = 8 * COLS_FS_CPT;
+        const int cells = s.N - 1, gx = (s.M + 3 + 63) / 64;
+        if (hyd)
+            LAUNCH(ctx, "k_hyd_assemble<uz>",
+                   (k_hyd_assemble_cols<COLS_HYD_CPT><<<dim3(gx, (cells + TRH - 1) / TRH, n), dim3(64, 8), 0, st>>>(
+                       S, B, start, end)));
+        else
+            LAUNCH(ctx, "k_fs_assemble<j>",
+                   (k_fs_assemble_cols<COLS_FS_CPT><<<dim3(gx, (cells + TRF - 1) / TRF, n), dim3(64, 8), 0, st>>>(
+                       S, B, start, end, t, s_start, from_clock)));
+        return HD_OK;
+    }
+    const dim3 tg = tile_grid(s, n);
+    if (hyd)
+        LAUNCH(ctx, DIR ? "k_hyd_assemble<uz>" : "k_hyd_assemble<vz>",
+               (k_hyd_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end)));
+    else
+        LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
+               (k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, from_clock)));
+    return HD_OK;
+}
+
 // Lets a kernel use more than the default 64 KiB of dynamic LDS (gfx950
 // has 160 KiB per CU).
 template <typename K>
@@ -1125,21 +1281,16 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = LAYOUT == ROW_MAJOR ? B.ld_rm[DIR] : B.ld_lm[DIR];
-    const dim3 tg = tile_grid(s, n);
-    if (hyd)
-        LAUNCH(ctx, DIR ? "k_hyd_assemble<uz>" : "k_hyd_assemble<vz>",
-               k_hyd_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end));
-    else
-        LAUNCH(ctx, DIR ? "k_fs_assemble<j>" : "k_fs_assemble<i>",
-               k_fs_assemble<DIR, LAYOUT><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, ctx->capturing ? 1 : 0));
+    int rc = launch_assembly<DIR, LAYOUT>(ctx, S, s, B, n, hyd, start, end, t, s_start, ctx->capturing ? 1 : 0);
+    if (rc) return rc;
     const int lines = end - start + 1;
     if (LAYOUT == ROW_MAJOR) {
         LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
     } else {
         const int h = hyd ? 1 : 0;
-        int rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
-                            nrows);
+        rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
+                        nrows);
         if (rc) return rc;
     }
     const Launch2D g = grid2d(s, n);
@@ -1162,12 +1313,8 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = B.ld_lm[DIR];
-    const dim3 tg = tile_grid(s, n);
-    if (hyd)
-        LAUNCH(ctx, "k_hyd_assemble", (k_hyd_assemble<DIR, LINE_MAJOR><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end)));
-    else
-        LAUNCH(ctx, "k_fs_assemble",
-               (k_fs_assemble<DIR, LINE_MAJOR><<<tg, dim3(64, 8), 0, st>>>(S, B, start, end, t, s_start, 0)));
+    int arc = launch_assembly<DIR, LINE_MAJOR>(ctx, S, s, B, n, hyd, start, end, t, s_start, 0);
+    if (arc) return arc;
     const int nz = hyd ? nrows / 2 : nrows;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return HD_E_HIP;
