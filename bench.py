@@ -77,8 +77,7 @@ def stage_bytes(name, sim, sc):
         "k_update_fs": 16,
         "k_qb": 7 * 8 + 4 + 16,
         "k_bedload": 12 * 8 + 4 + 16,
-        "k_segment_rows": 4,          # khouot in (segment tables are O(lines))
-        "k_segment_cols": 4,
+        "k_segment": 8,               # khouot in, once per direction (segment tables are O(lines))
         "k_dry": 8 + 4,
     }
     if name.startswith("k_rewet"):
@@ -135,6 +134,10 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
         return "k_hyd_assemble_cols<1>"
     if layout == 1 and stage == "k_fs_assemble<j>":
         return "k_fs_assemble_cols<1>"
+    if layout == 1 and stage == "k_hyd_extract<uz>":
+        return "k_hyd_extract_cols"
+    if layout == 1 and stage == "k_fs_extract<j>":
+        return "k_fs_extract_cols"
     for prefix in ("k_hyd_assemble<", "k_hyd_extract<", "k_fs_assemble<", "k_fs_extract<"):
         if stage.startswith(prefix) and layout is not None:
             what = stage[len(prefix):-1]

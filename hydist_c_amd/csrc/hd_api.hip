@@ -179,9 +179,9 @@ int prof_resolve(hd_ctx *ctx) {
 // ---------------------------------------------------------------------------
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n) {
     hipStream_t st = ctx->stream;
-    LAUNCH(ctx, "k_segment_rows", k_segment_rows<<<dim3((s.N - 1 + 3) / 4, n), 256, 0, st>>>(S));
-    LAUNCH(ctx, "k_segment_cols", k_segment_cols<<<dim3((s.M - 1 + SEG_COLS - 1) / SEG_COLS, n), SEG_COLS * SEG_GROUPS, 0, st>>>(S));
-    LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
+    const int rb = (s.N - 1 + SEG_ROWS_PER_BLOCK - 1) / SEG_ROWS_PER_BLOCK;
+    const int cb = (s.M - 1 + SEG_COLS - 1) / SEG_COLS;
+    LAUNCH(ctx, "k_segment", k_segment<<<dim3(rb + cb + 1, n), SEG_COLS * SEG_GROUPS, 0, st>>>(S, rb, cb));
     return HD_OK;
 }
 

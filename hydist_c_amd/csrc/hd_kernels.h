@@ -6,10 +6,12 @@
 namespace hd {
 
 __global__ void k_onetime_init(const DevState *S);
-__global__ void k_segment_rows(const DevState *S);
-// k_segment_cols: a workgroup of SEG_COLS x SEG_GROUPS threads per SEG_COLS columns
+// k_segment: workgroups of SEG_COLS x SEG_GROUPS threads; a row block takes
+// SEG_ROWS_PER_BLOCK rows (one per wavefront), a column block SEG_COLS
+// columns
 constexpr int SEG_COLS = 32, SEG_GROUPS = 32;
-__global__ void k_segment_cols(const DevState *S);
+constexpr int SEG_ROWS_PER_BLOCK = 4;
+__global__ void k_segment(const DevState *S, int row_blocks, int col_blocks);
 __global__ void k_htuongdoi(const DevState *S);
 __global__ void k_preprocess(const DevState *S);
 __global__ void k_boundary(const DevState *S, double t, int from_clock);

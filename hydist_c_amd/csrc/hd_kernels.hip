@@ -49,12 +49,14 @@ __global__ void k_onetime_init(const DevState *__restrict__ S) {
 // ---------------------------------------------------------------------------
 
 // Rows (lines along j, unit stride): one wavefront per row; each lane tests
-// one cell per chunk and ballots run starts / ends.
-__global__ void __launch_bounds__(256) k_segment_rows(const DevState *__restrict__ S) {
-    const DevState &s = S[blockIdx.y];
+// one cell per chunk and ballots run starts / ends.  Block b takes rows
+// 2 + b * SEG_ROWS_PER_BLOCK onwards, one per wavefront; its other
+// wavefronts have nothing to do (few rows per block spread the serial row
+// walks over all CUs).
+HD_DEV void segment_rows_block(const DevState &s, int b) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wave + 2;
-    if (i > s.N) return;
+    const int i = b * SEG_ROWS_PER_BLOCK + wave + 2;
+    if (wave >= SEG_ROWS_PER_BLOCK || i > s.N) return;
     const int end = s.M, W = s.P;
     const int *kh = s.khouot + i * W;
     int *first = s.segH_f + (size_t)i * s.capH, *last = s.segH_l + (size_t)i * s.capH;
@@ -86,12 +88,12 @@ __global__ void __launch_bounds__(256) k_segment_rows(const DevState *__restrict
 // SEG_GROUPS contiguous groups, one thread per (group, column).  Pass 1
 // counts run starts per group, an LDS prefix gives each group its first
 // segment index, pass 2 writes the run starts and ends.
-__global__ void __launch_bounds__(SEG_COLS *SEG_GROUPS) k_segment_cols(const DevState *__restrict__ S) {
-    const DevState &s = S[blockIdx.y];
+// Needs blockDim.x == SEG_COLS * SEG_GROUPS (every thread reaches the barrier).
+HD_DEV void segment_cols_block(const DevState &s, int b) {
     __shared__ int cnt[SEG_GROUPS][SEG_COLS];
     __shared__ int lastst[SEG_GROUPS][SEG_COLS];
     const int tx = threadIdx.x % SEG_COLS, g = threadIdx.x / SEG_COLS;
-    const int j = blockIdx.x * SEG_COLS + tx + 2;
+    const int j = b * SEG_COLS + tx + 2;
     const int end = s.N, W = s.P;
     const int rows = end - 1;  // rows 2..end
     const int per = (rows + SEG_GROUPS - 1) / SEG_GROUPS;
@@ -558,8 +560,7 @@ __global__ void k_post_b(const DevState *__restrict__ S) {
 
 // Interpolate_FS_ng on rows N and 2, then Interpolate_FS_d on column M
 // (support_funcs.cpp:191-237, called from the segmentation kernels).
-__global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__ S) {
-    const DevState &s = S[blockIdx.x];
+HD_DEV void interp_fs_block(const DevState &s) {
     const int M = s.M, N = s.N, W = s.P;
     const double dT = s.c.dT, dX = s.c.dX, dY = s.c.dY;
     double *FS = s.FS;
@@ -594,6 +595,25 @@ __global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__
             FS[i * W + location] = FS[i * W + j + 1] + (FS[i * W + j + 1] - FS[i * W + j]) * (deltaY / dY - j_deltaY / j_dY);
         }
     }
+}
+
+__global__ void __launch_bounds__(1024) k_interp_fs(const DevState *__restrict__ S) { interp_fs_block(S[blockIdx.x]); }
+
+// Segmentation of both directions and the FS boundary interpolation in one
+// launch (they touch disjoint data: khouot in, the segment tables out; FS
+// on the outflow rows/column): blocks [0, row_blocks) take SEG_ROWS_PER_BLOCK
+// rows each, the next col_blocks take SEG_COLS columns each, the last block
+// the interpolation.  blockIdx.y is the scenario.
+__global__ void __launch_bounds__(SEG_COLS *SEG_GROUPS) k_segment(const DevState *__restrict__ S, int row_blocks,
+                                                                   int col_blocks) {
+    const DevState &s = S[blockIdx.y];
+    const int b = blockIdx.x;
+    if (b < row_blocks)
+        segment_rows_block(s, b);
+    else if (b < row_blocks + col_blocks)
+        segment_cols_block(s, b - row_blocks);
+    else
+        interp_fs_block(s);
 }
 
 // ---------------------------------------------------------------------------

@@ -1040,25 +1040,16 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
 // Extraction (hydraulics).  Define: cells before `last` are written before
 // the end cell, whose continuity formula uses the new velocity at last-1.
 // ---------------------------------------------------------------------------
-template <int DIR, int LAYOUT>
-__global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
-    const SweepBufs B = Bin.scen(blockIdx.z);
-    const DevState &s = S[blockIdx.z];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y * blockDim.y + threadIdx.y;
-    const int line = DIR == 1 ? j : i;
-    const int p = DIR == 1 ? i : j;
+// Cell p of line `line`; xat(rho) reads the line's solution row rho.
+template <int DIR, typename XAt>
+HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
     const int L = DIR == 1 ? s.N : s.M;
-    if (line < start || line > end || p < 2 || p > L) return;
     int k, f, l;
     if (!locate<DIR>(s, line, p, k, f, l)) return;
     const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
     const SegInfo &g = sc.g;
-    const int W = s.P, ld = B.ld;
+    const int W = s.P;
     auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
-    auto xat = [&](int rho) {
-        return LAYOUT == ROW_MAJOR ? B.X[rm_idx(rho, line, ld)] : B.X[lm_idx(rho, line, ld)];
-    };
     double *__restrict__ tz = s.t_z;
     double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
     const int sn = g.sn;
@@ -1092,11 +1083,8 @@ __global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs Bin, int
     }
 }
 
-// FS extraction (sediment_transport.cpp:167-201, :316-349): negative values
-// become NDnen; the segment's end cells take the neighbouring value or keep
-// FS as the reference's end tests decide.
 template <int DIR, int LAYOUT>
-__global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
+__global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
     const SweepBufs B = Bin.scen(blockIdx.z);
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1105,15 +1093,26 @@ __global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs Bin, int 
     const int p = DIR == 1 ? i : j;
     const int L = DIR == 1 ? s.N : s.M;
     if (line < start || line > end || p < 2 || p > L) return;
+    const int ld = B.ld;
+    hyd_extract_cell<DIR>(s, line, p, [&](int rho) {
+        return LAYOUT == ROW_MAJOR ? B.X[rm_idx(rho, line, ld)] : B.X[lm_idx(rho, line, ld)];
+    });
+}
+
+// FS extraction (sediment_transport.cpp:167-201, :316-349): negative values
+// become NDnen; the segment's end cells take the neighbouring value or keep
+// FS as the reference's end tests decide.  xv() reads the solution at p.
+template <int DIR, typename XV>
+HD_DEV void fs_extract_cell(const DevState &s, int line, int p, XV xv) {
     int k, f, l;
     if (!locate<DIR>(s, line, p, k, f, l) || l - f < 2) return;
     if (p < f + 1 || p > l - 1) return;
     bool ran1, ran2;
     wall_flags(s, DIR, line, f, l, ran1, ran2);
-    const int W = s.P, ld = B.ld;
+    const int W = s.P;
     auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
-    const double xv = LAYOUT == ROW_MAJOR ? B.X[rm_idx(p - 2, line, ld)] : B.X[lm_idx(p - 2, line, ld)];
-    const double out = xv < 0 ? s.c.NDnen : xv;
+    const double x = xv();
+    const double out = x < 0 ? s.c.NDnen : x;
     const double *tvel = DIR == 1 ? s.t_u : s.t_v;
     s.tFS[cell(p)] = out;
     if (p == f + 1) {
@@ -1128,6 +1127,71 @@ __global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs Bin, int 
         else
             s.tFS[cell(l)] = s.FS[cell(l)];
     }
+}
+
+template <int DIR, int LAYOUT>
+__global__ void k_fs_extract(const DevState *__restrict__ S, SweepBufs Bin, int start, int end) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
+    const DevState &s = S[blockIdx.z];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    const int line = DIR == 1 ? j : i;
+    const int p = DIR == 1 ? i : j;
+    const int L = DIR == 1 ? s.N : s.M;
+    if (line < start || line > end || p < 2 || p > L) return;
+    const int ld = B.ld;
+    fs_extract_cell<DIR>(s, line, p, [&]() {
+        return LAYOUT == ROW_MAJOR ? B.X[rm_idx(p - 2, line, ld)] : B.X[lm_idx(p - 2, line, ld)];
+    });
+}
+
+// Column lines from [line][rho]: a tile of 64 lines x 8 cells along i
+// (i0 = 2 + 8 * blockIdx.y) first reads the rows its cells need, coalesced
+// along each line, into LDS: rows 2(i0-3) .. 2(i0+5)+1 for hydraulics (the
+// cells' own two rows and those of the cell before, which the segment-end
+// formulas read), rows i0-2 .. i0+5 for FS.  Rows outside that window are
+// read from global memory.  Same results as k_hyd_extract / k_fs_extract
+// with DIR = 1.
+constexpr int XT_ROWS = 18;
+
+__global__ void __launch_bounds__(512) k_hyd_extract_cols(const DevState *__restrict__ S, SweepBufs Bin, int start,
+                                                          int end, int nrows) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
+    const DevState &s = S[blockIdx.z];
+    __shared__ double xs[XT_ROWS][64 + 1];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * 64 + tx, ld = B.ld;
+    const int j0 = blockIdx.x * 64, i0 = 2 + blockIdx.y * 8;
+    const int rb = 2 * (i0 - 3);  // first row of the window (may be negative)
+    for (int e = t; e < 64 * XT_ROWS; e += 512) {
+        const int ln = e / XT_ROWS, r = e % XT_ROWS;  // r runs fastest: contiguous along the line
+        const int jj = j0 + ln, rho = rb + r;
+        if (jj >= start && jj <= end && rho >= 0 && rho < nrows) xs[r][ln] = B.X[lm_idx(rho, jj, ld)];
+    }
+    __syncthreads();
+    const int j = j0 + tx, i = i0 + ty;
+    if (j < start || j > end || i > s.N) return;
+    hyd_extract_cell<1>(s, j, i, [&](int rho) {
+        const int r = rho - rb;
+        return (r >= 0 && r < XT_ROWS) ? xs[r][tx] : B.X[lm_idx(rho, j, ld)];
+    });
+}
+
+__global__ void __launch_bounds__(512) k_fs_extract_cols(const DevState *__restrict__ S, SweepBufs Bin, int start,
+                                                         int end) {
+    const SweepBufs B = Bin.scen(blockIdx.z);
+    const DevState &s = S[blockIdx.z];
+    __shared__ double xs[8][64 + 1];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * 64 + tx, ld = B.ld;
+    const int j0 = blockIdx.x * 64, i0 = 2 + blockIdx.y * 8;
+    {
+        const int ln = t / 8, r = t % 8;  // 512 threads: one row each
+        const int jj = j0 + ln, ci = i0 + r;
+        if (jj >= start && jj <= end && ci <= s.N) xs[r][ln] = B.X[lm_idx(ci - 2, jj, ld)];
+    }
+    __syncthreads();
+    const int j = j0 + tx, i = i0 + ty;
+    if (j < start || j > end || i > s.N) return;
+    fs_extract_cell<1>(s, j, i, [&]() { return xs[ty][tx]; });
 }
 
 // ---------------------------------------------------------------------------
@@ -1292,6 +1356,15 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
                         nrows);
         if (rc) return rc;
+    }
+    if (DIR == 1 && LAYOUT == LINE_MAJOR) {
+        const dim3 xg((s.M + 3 + 63) / 64, (s.N - 1 + 7) / 8, n);
+        if (hyd)
+            LAUNCH(ctx, "k_hyd_extract<uz>",
+                   (k_hyd_extract_cols<<<xg, dim3(64, 8), 0, st>>>(S, B, start, end, nrows)));
+        else
+            LAUNCH(ctx, "k_fs_extract<j>", (k_fs_extract_cols<<<xg, dim3(64, 8), 0, st>>>(S, B, start, end)));
+        return HD_OK;
     }
     const Launch2D g = grid2d(s, n);
     if (hyd)
