@@ -213,7 +213,7 @@ def serving_measure(seconds, rate, seed=0, max_active=32, side=(128, 512), durat
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 10))
     arrivals = arrivals[arrivals < seconds]
-    reqs = [S.serving_request(k, side=side, duration=duration) for k in range(len(arrivals))]
+    reqs = [S.serving_request(k, side=side, duration=duration, fp64_form=1) for k in range(len(arrivals))]
     srv = engine.Server(hbm_budget=64 << 30, max_active=max_active, steps_per_quantum=8, use_graph=False)
     t0 = time.perf_counter()
     ids = []
@@ -229,8 +229,89 @@ def serving_measure(seconds, rate, seed=0, max_active=32, side=(128, 512), durat
     srv.close()
     st.update({"rate_per_s": rate, "window_s": seconds,
                "workload": f"meander+islands, side log-U[{side[0]},{side[1]}], duration log-U[{duration[0]:g},"
-                           f"{duration[1]:g}] s simulated, sediment p=0.5, FP32+refine p=0.5"})
+                           f"{duration[1]:g}] s simulated, sediment p=0.5, FP32+refine p=0.5 else FP64 (fast form)"})
     return st
+
+
+def serving_full(seconds, rate, max_active=32, hbm_budget=200 << 30, queue_cap=4, seed=0):
+    """The serving workload at its full size: Poisson arrivals at `rate`
+    requests/s for `seconds` s of what-if runs (meander + islands meshes,
+    side log-uniform in [128, 2048], simulated duration log-uniform in
+    [600, 21600] s, sediment on for half, FP32 + refine for half and FP64
+    (fast form) for the rest).  The offered load is far above what one GPU
+    can serve, so most requests are still waiting when the window closes.
+    A request's mesh is generated and submitted only while fewer than
+    `queue_cap` submitted runs wait for admission; the rest wait, in
+    arrival order, on the client side.  Latency runs from arrival (client
+    clock) to the poll that sees the run done.  At the end of the window
+    the unfinished runs are cancelled; reported: completions, p50/p99
+    latency of the completed runs, cell-steps served in the window
+    (completed runs plus the progress of the running ones) and the
+    backlog."""
+    import collections
+    import numpy as np
+    from hydist_c_amd import engine, scenarios as S
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=int(rate * seconds * 2) + 10))
+    arrivals = arrivals[arrivals < seconds]
+    srv = engine.Server(hbm_budget=hbm_budget, max_active=max_active, steps_per_quantum=8, use_graph=False)
+    backlog = collections.deque()
+    live = {}  # rid -> (arrival, cells, dT steps, Tmax)
+    lat, served, failed, nxt = [], 0.0, 0, 0
+    side_done = []
+    t0 = time.perf_counter()
+    last_log = 0.0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= seconds:
+            break
+        while nxt < len(arrivals) and arrivals[nxt] <= now:
+            backlog.append(nxt)
+            nxt += 1
+        waiting = 0
+        for rid, (at, cells, steps, T) in list(live.items()):
+            st, _ = srv.poll(rid)
+            if st == engine.RUN_DONE or st in (engine.RUN_FAILED, engine.RUN_CANCELLED):
+                if st == engine.RUN_DONE:
+                    lat.append(time.perf_counter() - t0 - at)
+                    served += cells * steps
+                    side_done.append(int(round(cells ** 0.5)))
+                else:
+                    failed += 1
+                srv.release(rid)
+                del live[rid]
+            elif st == engine.RUN_QUEUED:
+                waiting += 1
+        while backlog and waiting < queue_cap and time.perf_counter() - t0 < seconds:
+            k = backlog.popleft()
+            sc, T, form, sed = S.serving_request(k, fp64_form=1)
+            rid = srv.submit(sc, T, form=form, sediment=sed, bed_change=sed)
+            live[rid] = (float(arrivals[k]), sc.M * sc.N, int(np.ceil(T / 2.0)), T)
+            waiting += 1
+        if now - last_log >= 15.0:
+            last_log = now
+            log(f"[serving-full] t={now:5.0f}s arrived {nxt} completed {len(lat)} live {len(live)} "
+                f"client backlog {len(backlog)}")
+        time.sleep(0.02)
+    # progress of the runs still live, then cancel them
+    running = 0
+    for rid, (at, cells, steps, T) in live.items():
+        st, t = srv.poll(rid)
+        if st == engine.RUN_RUNNING:
+            running += 1
+            served += cells * min(steps, int(t / 2.0))
+        srv.cancel(rid)
+    srv.close()
+    lat = np.array(lat)
+    return {"window_s": seconds, "rate_per_s": rate, "arrived": int(nxt), "completed": int(len(lat)),
+            "failed": failed, "running_at_end": running, "backlog_at_end": int(nxt - len(lat) - failed),
+            "p50_s": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_s": float(np.percentile(lat, 99)) if len(lat) else None,
+            "cell_steps_served": served, "cell_steps_per_s": served / seconds,
+            "median_side_completed": float(np.median(side_done)) if side_done else None,
+            "max_active": max_active, "hbm_budget": hbm_budget,
+            "workload": "meander+islands, side log-U[128,2048], duration log-U[600,21600] s simulated, "
+                        "sediment p=0.5, FP32+refine p=0.5 else FP64 (fast form)"}
 
 
 def ensemble_measure(n_scen, steps, forms=("plain", "fast", "fp32")):
@@ -270,6 +351,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=10.0,
                     help="arrival window of the serving measurement (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--serving-full", type=float, default=0.0,
+                    help="window (s) of the full-size serving measurement, e.g. 600 (0 = skip)")
     ap.add_argument("--ensemble", type=int, default=0,
                     help="scenarios of the ensemble measurement, e.g. 256 (0 = skip)")
     ap.add_argument("--ensemble-steps", type=int, default=30)
@@ -357,6 +440,12 @@ def main():
         serving = serving_measure(args.serving_seconds, args.serving_rate)
         log(f"[bench] serving: p50 {serving['p50_s']:.3f} s, p99 {serving['p99_s']:.3f} s, "
             f"{serving['completed']}/{serving['submitted']} done, {serving['cell_steps_per_s']:.3e} cell-steps/s")
+    full = None
+    if args.serving_full > 0:
+        log(f"[bench] full serving workload: {args.serving_rate:.0f} req/s for {args.serving_full:.0f} s")
+        full = serving_full(args.serving_full, args.serving_rate)
+        log(f"[bench] full serving: {full['completed']}/{full['arrived']} done, p50 {full['p50_s']} s, "
+            f"p99 {full['p99_s']} s, {full['cell_steps_per_s']:.3e} cell-steps/s, backlog {full['backlog_at_end']}")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
@@ -390,6 +479,7 @@ def main():
         "tridiag_roofline": tri,
         "cpu_baseline": cpu,
         "serving": serving,
+        "serving_full": full,
         "ensemble": ensemble,
         "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / max(v[0], 1)} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],

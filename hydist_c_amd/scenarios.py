@@ -187,11 +187,27 @@ def meander_islands(M, N, seed=2, n_islands=30, amplitude=None, wavelength=None,
         cy = centre(cx) + rng.uniform(-1, 1) * max(half_width - max(a, b) - 10, 1.0)
         islands.append((cy, cx, b, a))  # (row centre, col centre, row radius, col radius)
 
+    # Only values below `cut` change the mesh: rho > 0 is all in_channel
+    # asks, and a shore depth of depth_bank + 0.3 * cut already exceeds
+    # depth_max.  Each island is therefore evaluated on the window of the
+    # (meshgrid) coordinates where its rho can be below `cut`; elsewhere rho
+    # stays +inf, which gives the same mesh, bit for bit, as evaluating
+    # every island everywhere.
+    cut = (depth_max - depth_bank) / 0.3 + 1.0
+
     def island_rho(y, x):
-        rho = np.full(np.broadcast(y, x).shape, np.inf)
+        y, x = np.broadcast_arrays(y, x)
+        rho = np.full(y.shape, np.inf)
+        ys, xs = y[:, 0], x[0, :]  # meshgrid coordinates: rows along y, columns along x
         for cy, cx, ry, rx in islands:
-            r = np.sqrt(((y - cy) / ry) ** 2 + ((x - cx) / rx) ** 2)
-            rho = np.minimum(rho, (r - 1.0) * min(ry, rx))
+            R = 1.0 + cut / min(ry, rx)
+            r0, r1 = np.searchsorted(ys, cy - R * ry, "left"), np.searchsorted(ys, cy + R * ry, "right")
+            c0, c1 = np.searchsorted(xs, cx - R * rx, "left"), np.searchsorted(xs, cx + R * rx, "right")
+            if r0 >= r1 or c0 >= c1:
+                continue
+            yy, xx = y[r0:r1, c0:c1], x[r0:r1, c0:c1]
+            r = np.sqrt(((yy - cy) / ry) ** 2 + ((xx - cx) / rx) ** 2)
+            rho[r0:r1, c0:c1] = np.minimum(rho[r0:r1, c0:c1], (r - 1.0) * min(ry, rx))
         return rho  # distance-like: < 0 inside an island
 
     def in_channel(y, x):
@@ -439,20 +455,20 @@ def tidal_flats(M, N, seed=6, shelf=(0.03, 0.16), channel_depth=3.0, drop=2.0, r
     )
 
 
-def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0.5, p_fp32=0.5):
+def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0.5, p_fp32=0.5, fp64_form=0):
     """What-if request k of the serving workload: a meander + islands mesh
     with a log-uniform side in `side` (the other side 0.5-1.0 of it), a
     log-uniform simulated duration in `duration` (s), sediment on with
     probability p_sediment, and FP32 + refine (the looser tolerance) with
-    probability p_fp32, else plain FP64.  Returns (scenario, Tmax, form,
-    sediment)."""
+    probability p_fp32, else FP64 in form `fp64_form` (0 plain, 1 fast).
+    Returns (scenario, Tmax, form, sediment)."""
     rng = np.random.default_rng(50_000 + k)
     M = int(round(np.exp(rng.uniform(np.log(side[0]), np.log(side[1])))))
     N = max(32, int(round(M * rng.uniform(0.5, 1.0))))
     T = float(np.exp(rng.uniform(np.log(duration[0]), np.log(duration[1]))))
     T = 2.0 * np.ceil(T / 2.0)
     sed = bool(rng.random() < p_sediment)
-    form = 2 if rng.random() < p_fp32 else 0
+    form = 2 if rng.random() < p_fp32 else fp64_form
     n_isl = max(1, int(30 * M * N / (2048 * 1024)))
     s = meander_islands(M, N, seed=k, n_islands=n_isl, Q=-4000.0 * N / 1024, hours=int(np.ceil(T / 3600.0)),
                         sediment_start=0.0, name=f"request{k}_{M}x{N}")
