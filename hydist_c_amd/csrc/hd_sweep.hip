@@ -5,8 +5,9 @@
 //             tridiagonal system (UVZSolver_multithread.cpp:110-470 for the
 //             hydraulics, sediment_transport.cpp:51-313 for FS);
 //   solve:    either Thomas in the reference's row order, one lane per line
-//             (bitwise FP64, UVZSolver_multithread.cpp:19-43), or cyclic
-//             reduction, one workgroup per line (fast forms);
+//             (bitwise FP64, UVZSolver_multithread.cpp:19-43), or, for the
+//             fast forms, one workgroup per line: the partition solver
+//             (k_part, default) or cyclic reduction (k_cr_reg);
 //   extract:  one thread per cell writes the solution back into t_z and
 //             t_u / t_v (UVZSolver_multithread.cpp:472-603) or tFS
 //             (sediment_transport.cpp:167-201, :316-349).
@@ -545,15 +546,17 @@ __global__ void __launch_bounds__(64) k_thomas(const DevState *__restrict__ S, S
 }
 
 // ---------------------------------------------------------------------------
-// Cyclic reduction, one workgroup per line ([line][rho]).
+// Workgroup-per-line solvers of the fast forms ([line][rho]).
 //
 // Hydraulics (INTERLEAVED): the velocity rows (odd rho) are first
 // eliminated into the z rows, giving the z-only system of the line
-// (strictly diagonally dominant), which is solved by cyclic reduction in
-// precision T; with REFINE, the FP64 residual of that system is solved once
-// more (reusing the stored elimination multipliers) and added.  The
-// velocities are then recovered in FP64 from their own rows.
-// FS (!INTERLEAVED): the line's system is solved directly.
+// (strictly diagonally dominant), which is solved in precision T; with
+// REFINE, the FP64 residual of that system is solved once more (reusing
+// the stored elimination) and added.  The velocities are then recovered in
+// FP64 from their own rows.  FS (!INTERLEAVED): the line's system is
+// solved directly.  cr_solve / cr_replay below are the cyclic reduction of
+// n rows held in LDS that both k_cr_reg (the whole line) and k_part (its
+// reduced system) use.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct CRShared {
