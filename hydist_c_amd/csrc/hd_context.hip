@@ -127,6 +127,7 @@ int alloc_shared(hd_ctx *ctx, int nscen) {
                                std::max((size_t)(s.M + 3) * B.ld_lm[1], (size_t)(s.N + 3) * B.ld_lm[0]));
     B.stride = (sw + 63) / 64 * 64;
     for (double **q : {&B.A, &B.B, &B.C, &B.D, &B.X, &B.Ap, &B.Bp}) A((void **)q, B.stride * nscen * sizeof(double));
+    A((void **)&B.code, B.stride * nscen);
     const size_t nsys = std::max((size_t)s.capV * (s.M + 3), (size_t)s.capH * (s.N + 3));
     B.sstride = nsys;
     A((void **)&B.sys0, nsys * nscen * sizeof(int));
@@ -162,7 +163,7 @@ size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
                                std::max((size_t)(M + 3) * round_up(rowsU, 64), (size_t)(N + 3) * round_up(rowsV, 64)));
     const size_t stride = (sw + 63) / 64 * 64;
     const size_t nsys = std::max(capV * (M + 3), capH * (N + 3));
-    b += 7 * stride * nscen * 8 + 2 * nsys * nscen * 4 + 64 + sizeof(DevState) * nscen;
+    b += 7 * stride * nscen * 8 + stride * nscen + 2 * nsys * nscen * 4 + 64 + sizeof(DevState) * nscen;
     return b;
 }
 

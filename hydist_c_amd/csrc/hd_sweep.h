@@ -12,12 +12,15 @@ namespace hd {
 enum { ROW_MAJOR = 0, LINE_MAJOR = 1 };
 
 // Device buffers of one line sweep: the assembled tridiagonal rows, the
-// solution and the Thomas scratch, plus each segment's first row and last
-// row index (sn) in the line's row space.
+// solution and the Thomas scratch, each segment's first row and last row
+// index (sn) in the line's row space, and (plain form, [rho][line]) each
+// row's place in its system (0 outside any, 1 first, 2 interior, 3 last,
+// 4 a one-row system).
 // A batch of scenarios keeps one set per scenario, `stride` doubles (and
 // `sstride` segment entries) apart.
 struct SweepBufs {
     double *A, *B, *C, *D, *X, *Ap, *Bp;
+    signed char *code;
     int *sys0, *sysn;
     int ld;          // leading dimension of the active layout
     int ld_rm[2];    // [dir]: lines per row in ROW_MAJOR
@@ -27,7 +30,7 @@ struct SweepBufs {
     __host__ __device__ SweepBufs scen(int k) const {
         SweepBufs b = *this;
         const size_t o = (size_t)k * stride, so = (size_t)k * sstride;
-        b.A += o; b.B += o; b.C += o; b.D += o; b.X += o; b.Ap += o; b.Bp += o;
+        b.A += o; b.B += o; b.C += o; b.D += o; b.X += o; b.Ap += o; b.Bp += o; b.code += o;
         b.sys0 += so; b.sysn += so;
         return b;
     }

@@ -75,6 +75,15 @@ struct Row {
 
 HD_DEV Row identity_row() { return Row{0.0, 1.0, 0.0, 0.0}; }
 
+// Place of local row `loc` in a system whose last local row is sn (the
+// plain solver's row codes): 0 outside, 1 first of several, 2 interior,
+// 3 last of several, 4 the only row.
+HD_DEV signed char row_code(int loc, int sn) {
+    if (sn < 0 || loc < 0 || loc > sn) return 0;
+    if (sn == 0) return 4;
+    return loc == 0 ? 1 : (loc == sn ? 3 : 2);
+}
+
 // The two rows owned by position p of its segment (see the layout rules in
 // the reference's bienrandau / bienlongdau and end overrides).
 template <int DIR>
@@ -136,11 +145,15 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
     const bool active = line >= start && line <= end && p >= 2 && p <= L;
     const double f4 = DIR == 1 ? 2 * s.c.g * s.c.dTchia2dX : 2 * s.c.g * s.c.dTchia2dY;
     Row rz = identity_row(), ru = identity_row();
+    signed char cz = 0, cu = 0;
     if (active) {
         int k, f, l;
         if (locate<DIR>(s, line, p, k, f, l)) {
             const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
             hyd_rows<DIR>(s, line, p, sc, f4, rz, ru);
+            const int base = sc.g.isB ? 2 * f : 2 * f + 1;  // as in hyd_rows
+            cz = row_code(2 * p - base, sc.g.sn);
+            cu = row_code(2 * p + 1 - base, sc.g.sn);
             if (p == f) {
                 const int cap = DIR == 1 ? s.capV : s.capH;
                 B.sys0[(size_t)line * cap + k] = sc.rho0;
@@ -167,13 +180,20 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
         }
         B.A[iz] = rz.a; B.B[iz] = rz.b; B.C[iz] = rz.c; B.D[iz] = rz.d;
         B.A[iu] = ru.a; B.B[iu] = ru.b; B.C[iu] = ru.c; B.D[iu] = ru.d;
+        if (LAYOUT == ROW_MAJOR) {
+            B.code[iz] = cz;
+            B.code[iu] = cu;
+        }
     } else {
         // tile: 8 (i) x 64 (j) cells, 2 rows each -> transpose so that the
         // runs along i become contiguous
         __shared__ double tA[8][129], tB[8][129], tC[8][129], tD[8][129];
+        __shared__ signed char tK[8][128];
         const int ty = threadIdx.y, tx = threadIdx.x;
         tA[ty][2 * tx] = rz.a; tB[ty][2 * tx] = rz.b; tC[ty][2 * tx] = rz.c; tD[ty][2 * tx] = rz.d;
         tA[ty][2 * tx + 1] = ru.a; tB[ty][2 * tx + 1] = ru.b; tC[ty][2 * tx + 1] = ru.c; tD[ty][2 * tx + 1] = ru.d;
+        tK[ty][2 * tx] = cz;
+        tK[ty][2 * tx + 1] = cu;
         __syncthreads();
         // write out: for DIR==1 & LINE_MAJOR: line = j (64 lines), rho from i (8 cells -> 16 rows)
         //            for DIR==0 & ROW_MAJOR:  line = i (8 lines), rho from j (64 cells -> 128 rows)
@@ -198,6 +218,7 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
                 if (ii < start || ii > end || cj < 2 || cj > L) continue;
                 const size_t o = rm_idx(2 * (cj - 2) + slot, ii, ld);
                 B.A[o] = tA[ln][r]; B.B[o] = tB[ln][r]; B.C[o] = tC[ln][r]; B.D[o] = tD[ln][r];
+                B.code[o] = tK[ln][r];
             }
         }
     }
@@ -219,11 +240,13 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
     const int L = DIR == 1 ? s.N : s.M;
     const bool active = line >= start && line <= end && p >= 2 && p <= L;
     Row r = identity_row();
+    signed char code = 0;
     if (active) {
         int k, f, l;
         if (locate<DIR>(s, line, p, k, f, l) && l - f >= 2) {
             bool ran1, ran2;
             wall_flags(s, DIR, line, f, l, ran1, ran2);
+            code = row_code(p - (f + 1), l - f - 2);
             if (p >= f + 1 && p <= l - 1) {
                 if (DIR == 1)
                     fs_row_j(s, p, line, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
@@ -248,10 +271,13 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
         if (!active) return;
         const size_t o = LAYOUT == ROW_MAJOR ? rm_idx(p - 2, line, B.ld) : lm_idx(p - 2, line, B.ld);
         B.A[o] = r.a; B.B[o] = r.b; B.C[o] = r.c; B.D[o] = r.d;
+        if (LAYOUT == ROW_MAJOR) B.code[o] = code;
     } else {
         __shared__ double tA[8][65], tB[8][65], tC[8][65], tD[8][65];
+        __shared__ signed char tK[8][64];
         const int ty = threadIdx.y, tx = threadIdx.x;
         tA[ty][tx] = r.a; tB[ty][tx] = r.b; tC[ty][tx] = r.c; tD[ty][tx] = r.d;
+        tK[ty][tx] = code;
         __syncthreads();
         const int t = ty * 64 + tx;
         const int i0 = blockIdx.y * 8, j0 = blockIdx.x * 64;
@@ -267,6 +293,7 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
             if (ii < start || ii > end || cj < 2 || cj > L) return;
             const size_t o = rm_idx(cj - 2, ii, B.ld);
             B.A[o] = tA[ln][rr]; B.B[o] = tB[ln][rr]; B.C[o] = tC[ln][rr]; B.D[o] = tD[ln][rr];
+            B.code[o] = tK[ln][rr];
         }
     }
 }
@@ -400,148 +427,117 @@ __global__ void __launch_bounds__(512) k_fs_assemble_cols(const DevState *__rest
 
 // ---------------------------------------------------------------------------
 // Thomas, one lane per line, rows of all lines in lockstep ([rho][line]).
-// Rows are read in blocks of TB with the next block's loads issued before
-// the current block's serial recurrence, so memory latency overlaps it.
-// Arithmetic: tridiag (UVZSolver_multithread.cpp:19-43), unchanged.
+// Each row carries its place in its system (B.code, written by the
+// assembly), so the recurrence needs no segment table: the first, interior
+// and last rows of tridiag differ only in which operands feed its two
+// divisions (selects, one division site for all lanes), and every load and
+// store of a full block of TB rows is unconditional, so the next block's
+// rows are in flight while the current block's serial recurrence runs.
+// Lanes past the last line redo the last line (identical values to the
+// same addresses).  Arithmetic: tridiag (UVZSolver_multithread.cpp:19-43),
+// unchanged: den is b, ep = a*Ap + b, or b + a*Ap (the same value).
 // ---------------------------------------------------------------------------
 constexpr int TB = 8;
 
-__global__ void __launch_bounds__(64) k_thomas(const DevState *__restrict__ S, SweepBufs Bin, int dir, int start,
-                                               int end, int nrows) {
+struct ThomasRow {
+    double a, b, c, d;
+    int code;
+};
+
+__global__ void __launch_bounds__(64) k_thomas(SweepBufs Bin, int start, int end, int nrows) {
     const SweepBufs B = Bin.scen(blockIdx.y);
-    const DevState &s = S[blockIdx.y];
-    const int line = start + blockIdx.x * 64 + threadIdx.x;
-    const bool active = line <= end;
+    const int line = min(start + (int)(blockIdx.x * 64 + threadIdx.x), end);
     const int ld = B.ld;
-    const int cap = dir == 1 ? s.capV : s.capH;
-    const int nseg = active ? (dir == 1 ? s.segV_n : s.segH_n)[line] : 0;
-    const int *sys0 = B.sys0 + (size_t)(active ? line : 0) * cap;
-    const int *sysn = B.sysn + (size_t)(active ? line : 0) * cap;
-    const double *__restrict__ A = B.A + (active ? line : 0);
-    const double *__restrict__ Bv = B.B + (active ? line : 0);
-    const double *__restrict__ C = B.C + (active ? line : 0);
-    const double *__restrict__ D = B.D + (active ? line : 0);
-    double *__restrict__ Ap = B.Ap + (active ? line : 0);
-    double *__restrict__ Bp = B.Bp + (active ? line : 0);
-    double *__restrict__ X = B.X + (active ? line : 0);
+    const double *__restrict__ A = B.A + line;
+    const double *__restrict__ Bv = B.B + line;
+    const double *__restrict__ C = B.C + line;
+    const double *__restrict__ D = B.D + line;
+    const signed char *__restrict__ K = B.code + line;
+    double *__restrict__ Ap = B.Ap + line;
+    double *__restrict__ Bp = B.Bp + line;
+    double *__restrict__ X = B.X + line;
+    const int nfull = nrows / TB;
+    auto load_row = [&](int rho) {
+        const size_t o = (size_t)min(max(rho, 0), nrows - 1) * ld;
+        return ThomasRow{A[o], Bv[o], C[o], D[o], (int)K[o]};
+    };
 
     // forward
-    int k = 0, r0 = 0, sn = -1;
-    auto next_sys = [&]() {
-        while (k < nseg && sysn[k] < 0) k++;
-        if (k < nseg) {
-            r0 = sys0[k];
-            sn = sysn[k];
-        } else {
-            r0 = nrows;
-            sn = -1;
-        }
-    };
-    next_sys();
     double ApP = 0, BpP = 0;
-    double ca[TB], cb[TB], cc[TB], cd[TB];
-    double na[TB], nb[TB], nc[TB], nd[TB];
-    for (int q = 0; q < TB; q++) {
-        const int rho = q;
-        const bool ok = active && rho < nrows;
-        na[q] = ok ? A[(size_t)rho * ld] : 0;
-        nb[q] = ok ? Bv[(size_t)rho * ld] : 1;
-        nc[q] = ok ? C[(size_t)rho * ld] : 0;
-        nd[q] = ok ? D[(size_t)rho * ld] : 0;
-    }
-    for (int blk = 0; blk < nrows; blk += TB) {
-        for (int q = 0; q < TB; q++) {
-            ca[q] = na[q]; cb[q] = nb[q]; cc[q] = nc[q]; cd[q] = nd[q];
-        }
-        for (int q = 0; q < TB; q++) {
-            const int rho = blk + TB + q;
-            const bool ok = active && rho < nrows;
-            na[q] = ok ? A[(size_t)rho * ld] : 0;
-            nb[q] = ok ? Bv[(size_t)rho * ld] : 1;
-            nc[q] = ok ? C[(size_t)rho * ld] : 0;
-            nd[q] = ok ? D[(size_t)rho * ld] : 0;
-        }
-        for (int q = 0; q < TB; q++) {
-            const int rho = blk + q;
-            if (rho >= nrows) break;
-            if (k < nseg && rho >= r0 && rho <= r0 + sn) {
-                const int r = rho - r0;
-                const double a = ca[q], b = cb[q], c = cc[q], d = cd[q];
-                double ap, bp;
-                if (r == 0) {
-                    if (sn == 0) {
-                        ap = 0;
-                        bp = d / b;
-                    } else {
-                        ap = -c / b;
-                        bp = d / b;
-                    }
-                } else if (r < sn) {
-                    const double ep = a * ApP + b;
-                    ap = -c / ep;
-                    bp = (d - (a * BpP)) / ep;
-                } else {
-                    ap = 0;
-                    bp = (d - (a * BpP)) / (b + (a * ApP));
-                }
-                Ap[(size_t)rho * ld] = ap;
-                Bp[(size_t)rho * ld] = bp;
-                ApP = ap;
-                BpP = bp;
-                if (r == sn) {
-                    k++;
-                    next_sys();
-                }
-            }
-        }
-    }
-    // backward
-    k = nseg - 1;
-    auto prev_sys = [&]() {
-        while (k >= 0 && sysn[k] < 0) k--;
-        if (k >= 0) {
-            r0 = sys0[k];
-            sn = sysn[k];
-        } else {
-            r0 = -1;
-            sn = -1;
+    auto fwd = [&](const ThomasRow &w, double &ap, double &bp) {
+        const bool first = w.code == 1 || w.code == 4;
+        const double den = first ? w.b : w.a * ApP + w.b;
+        const double num = first ? w.d : w.d - (w.a * BpP);
+        ap = (w.code == 1 || w.code == 2) ? -w.c / den : 0.0;
+        bp = num / den;
+        if (w.code != 0) {
+            ApP = ap;
+            BpP = bp;
         }
     };
-    prev_sys();
-    double xnext = 0;
-    const int top = ((nrows + TB - 1) / TB) * TB;
-    double pa[TB], pb[TB], qa[TB], qb[TB];
-    for (int q = 0; q < TB; q++) {
-        const int rho = top - 1 - q;
-        const bool ok = active && rho < nrows && rho >= 0;
-        qa[q] = ok ? Ap[(size_t)rho * ld] : 0;
-        qb[q] = ok ? Bp[(size_t)rho * ld] : 0;
+    ThomasRow cur[TB], nxt[TB];
+#pragma unroll
+    for (int q = 0; q < TB; q++) nxt[q] = load_row(q);
+    for (int blk = 0; blk < nfull; blk++) {
+#pragma unroll
+        for (int q = 0; q < TB; q++) cur[q] = nxt[q];
+#pragma unroll
+        for (int q = 0; q < TB; q++) nxt[q] = load_row((blk + 1) * TB + q);
+#pragma unroll
+        for (int q = 0; q < TB; q++) {
+            const size_t o = (size_t)(blk * TB + q) * ld;
+            double ap, bp;
+            fwd(cur[q], ap, bp);
+            Ap[o] = ap;
+            Bp[o] = bp;
+        }
     }
-    for (int blk = top - 1; blk >= 0; blk -= TB) {
+#pragma unroll
+    for (int q = 0; q < TB; q++) {  // rows past the full blocks
+        const int rho = nfull * TB + q;
+        if (rho >= nrows) break;
+        double ap, bp;
+        fwd(nxt[q], ap, bp);
+        Ap[(size_t)rho * ld] = ap;
+        Bp[(size_t)rho * ld] = bp;
+    }
+
+    // backward (rows outside any system get finite values nobody reads)
+    double xnext = 0;
+    auto bwd = [&](int code, double pa, double pb) {
+        const double xv = (code == 3 || code == 4) ? pb : pb + (pa * xnext);
+        xnext = xv;
+        return xv;
+    };
+    for (int rho = nrows - 1; rho >= nfull * TB; rho--) {
+        const size_t o = (size_t)rho * ld;
+        X[o] = bwd((int)K[o], Ap[o], Bp[o]);
+    }
+    double pa[TB], pb[TB], qa[TB], qb[TB];
+    int pk[TB], qk[TB];
+#pragma unroll
+    for (int q = 0; q < TB; q++) {
+        const size_t o = (size_t)max(nfull * TB - 1 - q, 0) * ld;
+        qa[q] = Ap[o];
+        qb[q] = Bp[o];
+        qk[q] = K[o];
+    }
+    for (int blk = nfull - 1; blk >= 0; blk--) {
+#pragma unroll
         for (int q = 0; q < TB; q++) {
             pa[q] = qa[q];
             pb[q] = qb[q];
+            pk[q] = qk[q];
         }
+#pragma unroll
         for (int q = 0; q < TB; q++) {
-            const int rho = blk - TB - q;
-            const bool ok = active && rho >= 0 && rho < nrows;
-            qa[q] = ok ? Ap[(size_t)rho * ld] : 0;
-            qb[q] = ok ? Bp[(size_t)rho * ld] : 0;
+            const size_t o = (size_t)max(blk * TB - 1 - q, 0) * ld;
+            qa[q] = Ap[o];
+            qb[q] = Bp[o];
+            qk[q] = K[o];
         }
-        for (int q = 0; q < TB; q++) {
-            const int rho = blk - q;
-            if (rho < 0) break;
-            if (k >= 0 && rho >= r0 && rho <= r0 + sn) {
-                const int r = rho - r0;
-                const double xv = (r == sn) ? pb[q] : pb[q] + (pa[q] * xnext);
-                X[(size_t)rho * ld] = xv;
-                xnext = xv;
-                if (r == 0) {
-                    k--;
-                    prev_sys();
-                }
-            }
-        }
+#pragma unroll
+        for (int q = 0; q < TB; q++) X[(size_t)(blk * TB + TB - 1 - q) * ld] = bwd(pk[q], pa[q], pb[q]);
     }
 }
 
@@ -1353,7 +1349,7 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     const int lines = end - start + 1;
     if (LAYOUT == ROW_MAJOR) {
         LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
-               k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(S, B, DIR, start, end, nrows));
+               k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
     } else {
         const int h = hyd ? 1 : 0;
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,

@@ -17,7 +17,8 @@ from parity_util import HYDRO, diff_report, make_pair
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-DRIVER = os.path.join(ROOT, "tests", "_build", "emu" if os.environ.get("HD_EMU") == "1" else "", "compat_driver")
+EMU = os.environ.get("HD_EMU") == "1"
+DRIVER = os.path.join(ROOT, "tests", "_build", "emu" if EMU else "", "compat_driver")
 
 
 def read_grid(path):
@@ -30,7 +31,7 @@ def test_compat_driver_real_river(tmp_path):
         os.makedirs(tmp_path / "Outputs" / d, exist_ok=True)
     print("[case] compat driver, Luy river, 5 min", flush=True)
     r = subprocess.run([DRIVER, "-f", inputs + "/", "-h", "0", "-m", "5", "-s", "0", "-i", "60"],
-                       cwd=str(tmp_path), capture_output=True, text=True, timeout=240)
+                       cwd=str(tmp_path), capture_output=True, text=True, timeout=1500 if EMU else 240)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     z_drv = read_grid(tmp_path / "Outputs" / "Z" / "z_300.txt")
     u_drv = read_grid(tmp_path / "Outputs" / "U" / "u_300.txt")
