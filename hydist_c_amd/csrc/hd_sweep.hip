@@ -437,7 +437,7 @@ __global__ void __launch_bounds__(512) k_fs_assemble_cols(const DevState *__rest
 // same addresses).  Arithmetic: tridiag (UVZSolver_multithread.cpp:19-43),
 // unchanged: den is b, ep = a*Ap + b, or b + a*Ap (the same value).
 // ---------------------------------------------------------------------------
-constexpr int TB = 8;
+constexpr int TB = 12;  // rows per block (measured: 8 -> 12 saves 5 %, 16 no better)
 
 struct ThomasRow {
     double a, b, c, d;
