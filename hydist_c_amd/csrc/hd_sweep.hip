@@ -849,31 +849,31 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     T *crb = reinterpret_cast<T *>(smem_part + 4 * nt);
     CRShared<T> sh{crb, crb + nt, crb + 2 * nt, crb + 3 * nt, crb + 4 * nt, crb + 5 * nt};
 
-    // Inbound: array a is read coalesced (element tid + m * nt) into
-    // registers while array a-1 goes through LDS; rows past nrows read as
-    // identity rows.
+    // Inbound: all four arrays are read coalesced (element tid + m * nt)
+    // into registers first, then go through LDS one after the other, so
+    // the later arrays' loads are in flight while the first ones are
+    // staged.  The loads are unconditional (rows past nrows read row
+    // nrows - 1 and are replaced by identity rows), which lets the
+    // compiler wait for each array's loads alone.
     double own[4][R], prv[4];
-    double cur[R], nxt[R];
-#pragma unroll
-    for (int m = 0; m < R; m++) {
-        const int e = tid + m * nt;
-        cur[m] = e < nrows ? B.A[lo + e] : 0.0;
-    }
+    double ld4[4][R];
 #pragma unroll
     for (int a = 0; a < 4; a++) {
-        if (a < 3) {
-            const double *nsrc = a == 0 ? B.B : (a == 1 ? B.C : B.D);
-            const double fill = a == 0 ? 1.0 : 0.0;
-#pragma unroll
-            for (int m = 0; m < R; m++) {
-                const int e = tid + m * nt;
-                nxt[m] = e < nrows ? nsrc[lo + e] : fill;
-            }
-        }
+        const double *src = a == 0 ? B.A : (a == 1 ? B.B : (a == 2 ? B.C : B.D));
+        const double fill = a == 1 ? 1.0 : 0.0;
 #pragma unroll
         for (int m = 0; m < R; m++) {
             const int e = tid + m * nt;
-            stage[e + e / R] = cur[m];
+            const double v = src[lo + min(e, nrows - 1)];
+            ld4[a][m] = e < nrows ? v : fill;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int m = 0; m < R; m++) {
+            const int e = tid + m * nt;
+            stage[e + e / R] = ld4[a][m];
         }
         __syncthreads();
 #pragma unroll
@@ -886,8 +886,6 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
             prv[a] = tid > 0 ? stage[e + e / R] : (a == 1 ? 1.0 : 0.0);
         }
         __syncthreads();
-#pragma unroll
-        for (int m = 0; m < R; m++) cur[m] = nxt[m];
     }
 
     // The thread's K rows (FP64) and, for hydraulics, its velocity rows.
