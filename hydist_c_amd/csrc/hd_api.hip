@@ -261,10 +261,13 @@ int run_halfstep_vz_b(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
         rc = run_line_sweep(ctx, S, s, n, 0, false, 3, s.N - 1, t, ops.bed_change_start);
         if (rc) return rc;
-        LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
         if (ctx->cfg.bed_change && t >= ops.bed_change_start) {
+            // Calculate_Qb does not read FS, so Update_FS can wait for (and
+            // run inside) the BedLoad pass
             LAUNCH(ctx, "k_qb", k_qb<<<g.grid, g.block, 0, st>>>(S));
-            LAUNCH(ctx, "k_bedload", k_bedload<<<g.grid, g.block, 0, st>>>(S));
+            LAUNCH(ctx, "k_bedload", k_bedload<true><<<g.grid, g.block, 0, st>>>(S));
+        } else {
+            LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
         }
     }
     return HD_OK;
@@ -582,7 +585,7 @@ int hd_bedload(hd_ctx *ctx, double t) {
     if (rc) return rc;
     Launch2D g = grid2d(ctx->hs, ctx->nscen);
     LAUNCH(ctx, "k_qb", k_qb<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
-    LAUNCH(ctx, "k_bedload", k_bedload<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
+    LAUNCH(ctx, "k_bedload", k_bedload<false><<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds));
     return HD_OK;
 }
 

@@ -29,7 +29,7 @@ __global__ void k_interp_fs(const DevState *S);
 __global__ void k_vth_k(const DevState *S);
 __global__ void k_update_fs(const DevState *S);
 __global__ void k_qb(const DevState *S);
-__global__ void k_bedload(const DevState *S);
+template <bool UPDATE_FS> __global__ void k_bedload(const DevState *S);
 __global__ void k_volume(const DevState *S);
 __global__ void k_watchdog(const DevState *S, int *flag);
 

@@ -701,10 +701,16 @@ __global__ void k_qb(const DevState *__restrict__ S) {
 // BedLoad -> _bed_load (sediment_transport.cpp:436-537, :664-673) on rows
 // i in [3, N-2] and row segments of at least three cells.  Q9: the downwind
 // test is t_u[pos] > 0.
+//
+// UPDATE_FS: Update_FS (FS = tFS on [1, N] x [1, M]) runs in the same pass;
+// the bed load then reads the new FS from tFS, which holds the same values,
+// so no thread reads a cell another thread is writing.
+template <bool UPDATE_FS>
 __global__ void k_bedload(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
+    if (UPDATE_FS && i >= 1 && j >= 1 && i <= s.N && j <= s.M) s.FS[i * s.P + j] = s.tFS[i * s.P + j];
     if (i < 3 || i > s.N - 2 || j < 2 || j > s.M) return;
     const int W = s.P, pos = i * W + j, M = s.M, N = s.N;
     const int n = s.segH_n[i];
@@ -715,7 +721,7 @@ __global__ void k_bedload(const DevState *__restrict__ S) {
     if (last - first < 2) return;
     const hd_coeffs &c = s.c;
     const double dX = c.dX, dY = c.dY, dT = c.dT;
-    const double *Qbx = s.Qbx, *Qby = s.Qby, *FS = s.FS;
+    const double *Qbx = s.Qbx, *Qby = s.Qby, *FS = UPDATE_FS ? s.tFS : s.FS;
     const int *kh = s.khouot;
     double p = 0, q = 0;
     if (kh[pos - W] == 1) {
@@ -759,6 +765,8 @@ __global__ void k_bedload(const DevState *__restrict__ S) {
         Sv = source_nchs(c, wsm, Zf, Uf, H, fsv);
     s.dH[pos] += dT / (1 - c.Dorong) * (Sv + tH);
 }
+template __global__ void k_bedload<false>(const DevState *);
+template __global__ void k_bedload<true>(const DevState *);
 
 // Domain volume over wet compute cells, (htaiz + z) * dX * dY: per-row
 // partial sums in column order, then rows in order (one workgroup).
