@@ -1,0 +1,45 @@
+"""bench.py's bookkeeping on the CPU: stage names map to the kernel symbols
+rocprofv3 reports (so the roofline line names the profiled kernel), and
+the algorithmic byte counts follow the stated per-position figures."""
+import types
+
+import bench
+
+
+SC = types.SimpleNamespace(M=2048, N=1024)
+
+
+def test_solver_stage_symbols_follow_the_tuned_choice():
+    chosen = ["part/4", "part/8", "cr/256", "cr/0"]
+    assert bench.kernel_symbol("k_part<f64,uz>", SC, chosen, 1) == "k_part<double, false, true, 4>"
+    assert bench.kernel_symbol("k_part<f64,vz>", SC, chosen, 1) == "k_part<double, false, true, 8>"
+    assert bench.kernel_symbol("k_part<f32r,vz>", SC, chosen, 1) == "k_part<float, true, true, 8>"
+    # cyclic reduction: 1023 unknowns over 256 threads -> 4 per thread
+    assert bench.kernel_symbol("k_cr_reg<f64,fs_j>", SC, chosen, 1) == "k_cr_reg<double, false, false, 4>"
+    # default workgroup rule (512 above 1024 unknowns): 2047 / 512 -> 4
+    assert bench.kernel_symbol("k_cr_reg<f64,fs_i>", SC, chosen, 1) == "k_cr_reg<double, false, false, 4>"
+    # without a tuned choice: the default K (4 up to 2048 unknowns)
+    assert bench.kernel_symbol("k_part<f64,vz>", SC, None, 1) == "k_part<double, false, true, 4>"
+
+
+def test_assembly_and_extraction_symbols_by_layout():
+    assert bench.kernel_symbol("k_hyd_assemble<uz>", SC, None, 1) == "k_hyd_assemble_cols<1>"
+    assert bench.kernel_symbol("k_hyd_assemble<vz>", SC, None, 1) == "k_hyd_assemble<0, 1>"
+    assert bench.kernel_symbol("k_hyd_assemble<uz>", SC, None, 0) == "k_hyd_assemble<1, 0>"
+    assert bench.kernel_symbol("k_fs_assemble<j>", SC, None, 1) == "k_fs_assemble_cols<1>"
+    assert bench.kernel_symbol("k_fs_extract<i>", SC, None, 0) == "k_fs_extract<0, 0>"
+    assert bench.kernel_symbol("k_hyd_extract<uz>", SC, None, 1) == "k_hyd_extract_cols"
+    assert bench.kernel_symbol("k_thomas<vz>", SC, None, 0) == "k_thomas"
+    assert bench.kernel_symbol("k_post_a", SC, None, 1) == "k_post_a"
+
+
+def test_stage_bytes():
+    # hydraulic solve: 80 B per line position, (M-1)(N-1) positions
+    pos = (SC.M - 1) * (SC.N - 1)
+    assert bench.stage_bytes("k_part<f64,uz>", None, SC) == 80.0 * pos
+    assert bench.stage_bytes("k_cr_reg<f32r,vz>", None, SC) == 80.0 * pos
+    assert bench.stage_bytes("k_part<f64,fs_i>", None, SC) == 40.0 * pos
+    assert bench.stage_bytes("k_hyd_assemble<uz>", None, SC) == 160.0 * pos
+    cells = (SC.M + 3) * (SC.N + 3)
+    assert bench.stage_bytes("k_segment", None, SC) == 8.0 * cells
+    assert bench.stage_bytes("no_such_kernel", None, SC) is None

@@ -54,3 +54,19 @@ def test_autotune_reports_and_keeps_results():
         assert np.max(np.abs(a.get(f) - b.get(f))) <= 1e-9 * max(1.0, np.abs(a.get(f)).max()), f
     for g in sims:
         g.close()
+
+
+def test_set_solver_rejects_bad_choices():
+    print("[case] set_solver arguments", flush=True)
+    sc = S.open_box(40, 28)
+    g = engine.Simulation(sc, form=abi.HD_FORM_FP64_FAST)
+    g.init()
+    for d, hyd, kind, param in ((2, 1, engine.SOLVER_PARTITION, 4), (1, 1, engine.SOLVER_PARTITION, 3),
+                                (1, 1, engine.SOLVER_CR, 96), (1, 0, engine.SOLVER_CR, 1024), (0, 1, 7, 0),
+                                (0, 1, engine.SOLVER_DEFAULT, 4)):
+        with pytest.raises(engine.HydistError):
+            engine.set_solver(g, d, hyd, kind, param)
+    engine.set_solver(g, 0, 1, engine.SOLVER_PARTITION, 8)
+    engine.set_solver(g, 1, 1, engine.SOLVER_DEFAULT, 0)
+    g.step(2)
+    g.close()
