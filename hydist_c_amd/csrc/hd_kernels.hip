@@ -48,34 +48,53 @@ __global__ void k_onetime_init(const DevState *__restrict__ S) {
 // number of segments per line.
 // ---------------------------------------------------------------------------
 
-// Rows (lines along j, unit stride): one wavefront per row; each lane tests
-// one cell per chunk and ballots run starts / ends.  Block b takes rows
-// 2 + b * SEG_ROWS_PER_BLOCK onwards, one per wavefront; its other
-// wavefronts have nothing to do (few rows per block spread the serial row
+// Rows (lines along j, unit stride): one wavefront per row; lane l takes
+// position 2 + 64c + l of chunk c.  The khouot values of SEG_TILE chunks
+// (and of the position after them) are loaded at once, unconditionally
+// (clamped to the row), so one memory latency covers a whole tile; each
+// chunk's wet ballot then gives its run starts (wet, previous position dry
+// or p == 2) and ends (wet, next position dry or p == end) by bit shifts,
+// and the lanes write them in order through a popcount prefix.  Block b
+// takes rows 2 + b * SEG_ROWS_PER_BLOCK onwards, one per wavefront; its
+// other wavefronts have nothing to do (few rows per block spread the row
 // walks over all CUs).
+constexpr int SEG_TILE = 8;
+
 HD_DEV void segment_rows_block(const DevState &s, int b) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = b * SEG_ROWS_PER_BLOCK + wave + 2;
     if (wave >= SEG_ROWS_PER_BLOCK || i > s.N) return;
     const int end = s.M, W = s.P;
-    const int *kh = s.khouot + i * W;
+    const int *__restrict__ kh = s.khouot + i * W;
     int *first = s.segH_f + (size_t)i * s.capH, *last = s.segH_l + (size_t)i * s.capH;
     int nstart = 0, nend = 0, last_start = -1;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int base = 2; base <= end; base += 64) {
-        const int p = base + lane;
-        bool st = false, en = false;
-        if (p <= end) {
-            const bool wet = kh[p] == 0;
-            st = wet && (p == 2 || kh[p - 1] != 0);
-            en = wet && (p == end || kh[p + 1] != 0);
+    const int nch = (end - 2) / 64 + 1;  // chunks covering positions 2..end
+    uint64_t prevbit = 0;                 // wet bit of the position before the chunk (none before p = 2)
+    for (int c0 = 0; c0 < nch; c0 += SEG_TILE) {
+        int v[SEG_TILE];
+#pragma unroll
+        for (int u = 0; u < SEG_TILE; u++) v[u] = kh[min(2 + (c0 + u) * 64 + lane, end)];
+        const int pn = 2 + (c0 + SEG_TILE) * 64;  // first position after the tile
+        const int vn = kh[min(pn, end)];
+        uint64_t w[SEG_TILE];
+#pragma unroll
+        for (int u = 0; u < SEG_TILE; u++) w[u] = __ballot(2 + (c0 + u) * 64 + lane <= end && v[u] == 0);
+        const uint64_t wn = (pn <= end && vn == 0) ? 1ull : 0ull;
+#pragma unroll
+        for (int u = 0; u < SEG_TILE; u++) {
+            if (c0 + u >= nch) break;
+            const int base = 2 + (c0 + u) * 64, p = base + lane;
+            const uint64_t nxt = u + 1 < SEG_TILE ? (w[u + 1] & 1ull) : wn;
+            const uint64_t bs = w[u] & ~((w[u] << 1) | prevbit);
+            const uint64_t be = w[u] & ~((w[u] >> 1) | (nxt << 63));
+            prevbit = w[u] >> 63;
+            if ((bs >> lane) & 1ull) first[nstart + __popcll(bs & below)] = p;
+            if ((be >> lane) & 1ull) last[nend + __popcll(be & below)] = p;
+            nstart += __popcll(bs);
+            nend += __popcll(be);
+            if (bs) last_start = base + 63 - __clzll(bs);
         }
-        const uint64_t bs = __ballot(st), be = __ballot(en);
-        if (st) first[nstart + __popcll(bs & below)] = p;
-        if (en) last[nend + __popcll(be & below)] = p;
-        nstart += __popcll(bs);
-        nend += __popcll(be);
-        if (bs) last_start = base + 63 - __clzll(bs);
     }
     // a run starting at end-1 or end is not a segment; it is always the last run
     int n = nstart;
@@ -87,7 +106,9 @@ HD_DEV void segment_rows_block(const DevState &s, int b) {
 // columns (one cache line of khouot per row) and splits the rows 2..N into
 // SEG_GROUPS contiguous groups, one thread per (group, column).  Pass 1
 // counts run starts per group, an LDS prefix gives each group its first
-// segment index, pass 2 writes the run starts and ends.
+// segment index, pass 2 writes the run starts and ends.  Both passes load
+// SEG_TILE rows at once (clamped to the column), so one memory latency
+// covers a batch of rows.
 // Needs blockDim.x == SEG_COLS * SEG_GROUPS (every thread reaches the barrier).
 HD_DEV void segment_cols_block(const DevState &s, int b) {
     __shared__ int cnt[SEG_GROUPS][SEG_COLS];
@@ -99,17 +120,26 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
     const int per = (rows + SEG_GROUPS - 1) / SEG_GROUPS;
     const int r0 = 2 + g * per, r1 = min(end + 1, r0 + per);  // [r0, r1)
     const bool live = j <= s.M;
-    const int *kh = s.khouot;
+    const int *__restrict__ kh = s.khouot;
     int n = 0, ls = -1;
     if (live && r0 < r1) {
         bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
-        for (int p = r0; p < r1; p++) {
-            const bool wet = kh[p * W + j] == 0;
-            if (wet && !prev) {
-                n++;
-                ls = p;
+        for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
+            int v[SEG_TILE];
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++) v[u] = kh[min(p0 + u, r1 - 1) * W + j];
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++) {
+                const int p = p0 + u;
+                if (p < r1) {
+                    const bool wet = v[u] == 0;
+                    if (wet && !prev) {
+                        n++;
+                        ls = p;
+                    }
+                    prev = wet;
+                }
             }
-            prev = wet;
         }
     }
     cnt[g][tx] = n;
@@ -121,13 +151,21 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
         int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
         int k = off;  // starts at rows <= p
         bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
-        bool wet = kh[r0 * W + j] == 0;
-        for (int p = r0; p < r1; p++) {
-            const bool nwet = p < end && kh[(p + 1) * W + j] == 0;
-            if (wet && !prev) first[k++] = p;
-            if (wet && !nwet) last[k - 1] = p;
-            prev = wet;
-            wet = nwet;
+        for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
+            int v[SEG_TILE + 1];  // rows p0 .. p0 + SEG_TILE (the last for the end test)
+#pragma unroll
+            for (int u = 0; u <= SEG_TILE; u++) v[u] = kh[min(p0 + u, end) * W + j];
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++) {
+                const int p = p0 + u;
+                if (p < r1) {
+                    const bool wet = v[u] == 0;
+                    const bool nwet = p < end && v[u + 1] == 0;
+                    if (wet && !prev) first[k++] = p;
+                    if (wet && !nwet) last[k - 1] = p;
+                    prev = wet;
+                }
+            }
         }
     }
     if (g == 0 && live) {

@@ -135,3 +135,19 @@ def test_edge_mesh_fast_forms(form, mesh):
         a, b = g.get(f), o.get(f)
         scale = np.abs(b).max() + 1e-300
         assert np.max(np.abs(a - b)) <= 1e3 * TOL[form] * scale, (f, np.max(np.abs(a - b)), scale)
+
+
+def test_segments_of_a_wide_speckled_mesh():
+    """Segment tables after init on a mesh wide and tall enough that the
+    segmentation kernel takes several load batches per row and column
+    (rows of 1300 positions, columns of 700), equal to the oracle's."""
+    sc = speckled(1300, 700, 0.3, 21)
+    print(f"[case] edge-segments {sc.name}", flush=True)
+    g, o = make_pair(sc, seed_init=False)
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for d in (0, 1):
+        cg, fg, lg = g.segments(d)
+        co, fo, lo = o.segments(d)
+        assert fo.size > 10000
+        assert np.array_equal(cg, co) and np.array_equal(fg, fo) and np.array_equal(lg, lo)
+
