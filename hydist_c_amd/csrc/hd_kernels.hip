@@ -399,7 +399,14 @@ __global__ void k_solve_u(const DevState *__restrict__ S) {
 }
 
 // update_margin_elem_V / _U (UVZSolver_multithread.cpp:980-1060): one
-// thread per line walks its segments in order.
+// thread per line.  The reference walks the segments in order; only two of
+// its writes can read the velocity: the first segment's start (a later
+// segment starts past row/column 2, so its wall test holds and it writes
+// 0) and the last segment's end (an earlier segment ends before `end`,
+// so it writes 0).  Those two run first and last, and the zeros in
+// between, at distinct cells (segments are separated by dry cells), are
+// written from segment bounds loaded eight at a time, so the walk is not
+// a chain of memory latencies.  The results are those of the ordered walk.
 template <int DIR>
 __global__ void k_margin(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.y];
@@ -410,44 +417,58 @@ __global__ void k_margin(const DevState *__restrict__ S) {
     const double ND = s.c.NANGDAY;
     const double *h = s.h;
     const int n = DIR == 1 ? s.segV_n[line] : s.segH_n[line];
+    if (n <= 0) return;
     const int cap = DIR == 1 ? s.capV : s.capH;
     const int *fs = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)line * cap;
     const int *ls = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)line * cap;
     double *tv = DIR == 1 ? s.t_u : s.t_v;
-    for (int k = 0; k < n; k++) {
-        const int first = fs[k], last = ls[k];
-        bool ran1 = false, ran2 = false;
+    // cell q of the line
+    auto at = [&](int q) { return DIR == 1 ? (size_t)q * W + line : (size_t)line * W + q; };
+    {
+        const int first = fs[0];
+        bool ran1;
+        if (DIR == 1) {
+            const int j = line;
+            ran1 = (first > 2) || ((first == 2) && ((fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - ND) < 1e-16));
+        } else {
+            const int i = line;
+            ran1 = (first > 2) ||
+                   ((first == 2) && (fabs((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 - ND) < 1e-16));
+        }
+        if (ran1)
+            tv[at(first - 1)] = 0;
+        else
+            tv[at(first - 1)] = 2 * tv[at(first)] - tv[at(first + 1)];
+    }
+    for (int k0 = 0; k0 < n; k0 += 8) {
+        int f[8], l[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = min(k0 + u, n - 1);
+            f[u] = fs[k];
+            l[u] = ls[k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + u;
+            if (k > 0 && k < n) tv[at(f[u] - 1)] = 0;  // start behind a wall
+            if (k < n - 1) tv[at(l[u])] = 0;           // end before `end`
+        }
+    }
+    {
+        const int last = ls[n - 1];
+        bool ran2;
         if (DIR == 1) {
             const int j = line, N = s.N;
-            if ((first > 2) || ((first == 2) && ((fabs(h[1 * W + j] + h[1 * W + j - 1]) * 0.5 - ND) < 1e-16)))
-                ran1 = true;
-            if ((last < N) || ((last == N) && (fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - ND) < 1e-16)))
-                ran2 = true;
-            if (ran1)
-                tv[(first - 1) * W + j] = 0;
-            else
-                tv[(first - 1) * W + j] = 2 * tv[first * W + j] - tv[(first + 1) * W + j];
-            if (ran2)
-                tv[last * W + j] = 0;
-            else
-                tv[last * W + j] = 2 * tv[(last - 1) * W + j] - tv[(last - 2) * W + j];
+            ran2 = (last < N) || ((last == N) && (fabs((h[N * W + j] + h[N * W + j - 1]) * 0.5 - ND) < 1e-16));
         } else {
             const int i = line, M = s.M;
-            if ((first > 2) ||
-                ((first == 2) && (fabs((h[i * W + first - 1] + h[(i - 1) * W + first - 1]) * 0.5 - ND) < 1e-16)))
-                ran1 = true;
-            if ((last < M) ||
-                ((last == M) && (fabs((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 - ND) < 1e-16)))
-                ran2 = true;
-            if (ran1)
-                tv[i * W + first - 1] = 0;
-            else
-                tv[i * W + first - 1] = 2 * tv[i * W + first] - tv[i * W + first + 1];
-            if (ran2)
-                tv[i * W + last] = 0;
-            else
-                tv[i * W + last] = 2 * tv[i * W + last - 1] - tv[i * W + last - 2];
+            ran2 = (last < M) || ((last == M) && (fabs((h[i * W + last] + h[(i - 1) * W + last]) * 0.5 - ND) < 1e-16));
         }
+        if (ran2)
+            tv[at(last)] = 0;
+        else
+            tv[at(last)] = 2 * tv[at(last - 1)] - tv[at(last - 2)];
     }
 }
 template __global__ void k_margin<0>(const DevState *);
