@@ -272,51 +272,63 @@ __global__ void __launch_bounds__(1024) k_boundary(const DevState *__restrict__ 
     const int t1 = (int)(t / 3600);
     const double t2 = (t - (3600.0f * t1)) / 3600.0f;
     const double t2fs = (t - (3600.0 * t1)) / 3600.0;
-    const int k0 = min(t1, s.total_time - 1), k1 = min(t1 + 1, s.total_time - 1);
+    // series index clamped to the series (a time before 0 reads its first sample)
+    const int k0 = max(0, min(t1, s.total_time - 1)), k1 = max(0, min(t1 + 1, s.total_time - 1));
     const int imax = max(M, N);
+    // every side's segment-0 bounds and interpolated values first: nothing
+    // below writes them, so their loads overlap instead of opening each side
+    int n[4], f0[4], l0[4];
+    double value[4], fsval[4];
+#pragma unroll
     for (int side = 0; side < 4; side++) {
-        int n, f0, l0;
+        const int *sn = side < 2 ? s.segV_n : s.segH_n;
+        const int *sf = side < 2 ? s.segV_f : s.segH_f;
+        const int *sl = side < 2 ? s.segV_l : s.segH_l;
+        const size_t cap = side < 2 ? s.capV : s.capH;
+        const int line = side == 0 ? M : (side == 3 ? N : 2);
+        n[side] = sn[line];
+        f0[side] = sf[(size_t)line * cap];
+        l0[side] = sl[(size_t)line * cap];
+        const double *bc = s.bc[side];
+        value[side] = bc[k0] * (1.0f - t2) + bc[k1] * t2;
+        fsval[side] = s.hasCC[side] ? s.CC[side][k0] * (1.0 - t2fs) + s.CC[side][k1] * t2fs : 0.0;
+    }
+    for (int side = 0; side < 4; side++) {
         const double *hs;
         double *barr;
         bool isU;
         int loc, ext;
         if (side == 0) {
-            n = s.segV_n[M]; f0 = s.segV_f[(size_t)M * s.capV]; l0 = s.segV_l[(size_t)M * s.capV];
             hs = s.hi; barr = s.vbt; isU = false; loc = M; ext = M + 1;
         } else if (side == 1) {
-            n = s.segV_n[2]; f0 = s.segV_f[(size_t)2 * s.capV]; l0 = s.segV_l[(size_t)2 * s.capV];
             hs = s.hi + (N + 3); barr = s.vbd; isU = false; loc = 2; ext = 1;
         } else if (side == 2) {
-            n = s.segH_n[2]; f0 = s.segH_f[(size_t)2 * s.capH]; l0 = s.segH_l[(size_t)2 * s.capH];
             hs = s.hi + 2 * (N + 3); barr = s.ubt; isU = true; loc = 2; ext = 1;
         } else {
-            n = s.segH_n[N]; f0 = s.segH_f[(size_t)N * s.capH]; l0 = s.segH_l[(size_t)N * s.capH];
             hs = s.hi + 2 * (N + 3) + (M + 3); barr = s.ubp; isU = true; loc = N; ext = N + 1;
         }
-        if (n > 0) {
-            const double *bc = s.bc[side];
-            const double value = bc[k0] * (1.0f - t2) + bc[k1] * t2;
-            double fsval = 0.0;
-            if (s.hasCC[side]) fsval = s.CC[side][k0] * (1.0 - t2fs) + s.CC[side][k1] * t2fs;
+        if (n[side] > 0) {
+            const double v = value[side], fv = fsval[side];
+            const int a = f0[side], b = l0[side];
             const bool isQ = s.btype[side] == 2;
             for (int i = 2 + threadIdx.x; i <= imax; i += blockDim.x) {
-                if (!(f0 <= i && i <= l0)) continue;
+                if (!(a <= i && i <= b)) continue;
                 if (isQ) {
-                    barr[i] = value * hs[i];
+                    barr[i] = v * hs[i];
                     const int fi = isU ? loc : i, fj = isU ? i : loc;
-                    s.FS[fi * W + fj] = fsval * s.htaiz_bd[fi * W + fj] * s.hmax[side];
+                    s.FS[fi * W + fj] = fv * s.htaiz_bd[fi * W + fj] * s.hmax[side];
                 } else if (isU) {
-                    s.t_z[loc * W + i] = value;
-                    s.t_z[ext * W + i] = value;
+                    s.t_z[loc * W + i] = v;
+                    s.t_z[ext * W + i] = v;
                 } else {
-                    s.t_z[i * W + loc] = value;
-                    s.t_z[i * W + ext] = value;
+                    s.t_z[i * W + loc] = v;
+                    s.t_z[i * W + ext] = v;
                 }
             }
             if (isQ && threadIdx.x == 0) {
                 // prescribed discharge volume of this half-step (up/right sides face outward)
                 const double sign = (side == 0 || side == 3) ? -1.0 : 1.0;
-                s.acc[0] += sign * value * (0.5 * s.c.dT);
+                s.acc[0] += sign * v * (0.5 * s.c.dT);
             }
         }
         __syncthreads();
