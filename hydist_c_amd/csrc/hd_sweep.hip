@@ -178,9 +178,17 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
             iz = lm_idx(rz_i, line, ld);
             iu = lm_idx(ru_i, line, ld);
         }
-        B.A[iz] = rz.a; B.B[iz] = rz.b; B.C[iz] = rz.c; B.D[iz] = rz.d;
-        B.A[iu] = ru.a; B.B[iu] = ru.b; B.C[iu] = ru.c; B.D[iu] = ru.d;
-        if (LAYOUT == ROW_MAJOR) {
+        if (LAYOUT == LINE_MAJOR) {
+            // rows 2(p-2) and 2(p-2)+1 are adjacent: one 16-byte store per
+            // array (iz is even and ld, the scenario stride and the buffers
+            // are multiples of 64 elements, so the pair is 16-byte aligned)
+            *reinterpret_cast<double2 *>(B.A + iz) = make_double2(rz.a, ru.a);
+            *reinterpret_cast<double2 *>(B.B + iz) = make_double2(rz.b, ru.b);
+            *reinterpret_cast<double2 *>(B.C + iz) = make_double2(rz.c, ru.c);
+            *reinterpret_cast<double2 *>(B.D + iz) = make_double2(rz.d, ru.d);
+        } else {
+            B.A[iz] = rz.a; B.B[iz] = rz.b; B.C[iz] = rz.c; B.D[iz] = rz.d;
+            B.A[iu] = ru.a; B.B[iu] = ru.b; B.C[iu] = ru.c; B.D[iu] = ru.d;
             B.code[iz] = cz;
             B.code[iu] = cu;
         }
@@ -354,13 +362,16 @@ __global__ void __launch_bounds__(512) k_hyd_assemble_cols(const DevState *__res
         }
         __syncthreads();
         double *o0 = pass ? B.C : B.A, *o1 = pass ? B.D : B.B;
-        for (int e = t; e < 64 * 2 * TR; e += 512) {
-            const int ln = e / (2 * TR), r = e % (2 * TR);  // r runs fastest: contiguous in [line][rho]
-            const int jj = blockIdx.x * 64 + ln, ci = i0 + r / 2;
+        // each thread writes cell ci's two rows (z, velocity) of one line
+        // as one 16-byte store per array: 2(ci-2) is even and ld, the
+        // scenario stride and the buffers are multiples of 64 elements
+        for (int e = t; e < 64 * TR; e += 512) {
+            const int ln = e / TR, c = e % TR;  // c runs fastest: contiguous in [line][rho]
+            const int jj = blockIdx.x * 64 + ln, ci = i0 + c;
             if (jj < start || jj > end || ci > L) continue;
-            const size_t o = lm_idx(2 * (ci - 2) + (r & 1), jj, ld);
-            o0[o] = t0[r / 2][2 * ln + (r & 1)];
-            o1[o] = t1[r / 2][2 * ln + (r & 1)];
+            const size_t o = lm_idx(2 * (ci - 2), jj, ld);
+            *reinterpret_cast<double2 *>(o0 + o) = make_double2(t0[c][2 * ln], t0[c][2 * ln + 1]);
+            *reinterpret_cast<double2 *>(o1 + o) = make_double2(t1[c][2 * ln], t1[c][2 * ln + 1]);
         }
         if (pass == 0) __syncthreads();
     }

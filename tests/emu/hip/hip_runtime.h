@@ -11,6 +11,8 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -27,6 +29,25 @@ struct dim3 {
     unsigned x = 1, y = 1, z = 1;
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
+
+// 16-byte vector of the device code's paired stores; a store through a
+// double2 pointer that is not 16-byte aligned aborts with a message
+struct alignas(16) double2 {
+    double x, y;
+    double2() = default;
+    double2(double a, double b) : x(a), y(b) {}
+    double2(const double2 &) = default;
+    double2 &operator=(const double2 &o) {
+        if (reinterpret_cast<uintptr_t>(this) & 15) {
+            std::fprintf(stderr, "emu: misaligned double2 store at %p\n", (void *)this);
+            std::abort();
+        }
+        x = o.x;
+        y = o.y;
+        return *this;
+    }
+};
+inline double2 make_double2(double x, double y) { return double2(x, y); }
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNoDevice = 100 };
