@@ -263,6 +263,16 @@ HD_DEV double u_solver(const DevState &s, const double *__restrict__ t_v, int ro
 // ---------------------------------------------------------------------------
 // Sediment cell terms (sediment_transport.cpp:11-49).
 // ---------------------------------------------------------------------------
+// (1 - FS)^4 of the hindered settling velocity, wss * pow(1 - FS, 4)
+// (sediment_transport.cpp:94, :243, :524), as two squarings: within two
+// ulp of pow and a few times cheaper than the general FP64 pow.  FS and dH
+// are compared with the oracle to relative tolerances in any case (the
+// device's pow and log differ from the host's in the last bit).
+HD_DEV double pow4(double x) {
+    const double x2 = x * x;
+    return x2 * x2;
+}
+
 HD_DEV double source_chs(const hd_coeffs &c, double wsm, double Zf, double fs, double fw,
                          double vth, double h_moi) {
     double toee = c.Toe;
@@ -303,7 +313,7 @@ HD_DEV FSCell fs_cell(const DevState &s, int pos, double t, double s_start) {
     const double H = s.H_moi[pos], fsv = s.FS[pos], vth = s.VTH[pos];
     double cc = 18 * log(12 * H / c.Ks);
     double Uf = sqrt(c.g) * fabs(vth) / cc;
-    r.wsm = c.wss * pow((1 - fsv), 4);
+    r.wsm = c.wss * pow4(1 - fsv);
     r.Zf = 0;
     if (fsv > 0.00001) r.Zf = r.wsm / (0.4 * (Uf + 2 * r.wsm));
     if (s.ketdinh)

@@ -827,7 +827,7 @@ __global__ void k_bedload(const DevState *__restrict__ S) {
     const double H = s.H_moi[pos], vth = s.VTH[pos], fsv = FS[pos];
     double cc = 18 * log(12 * H / c.Ks);
     double Uf = sqrt(c.g) * fabs(vth) / cc;
-    double wsm = c.wss * pow((1 - fsv), 4);
+    double wsm = c.wss * pow4(1 - fsv);
     double Zf = 0, Sv = 0;
     if (fsv > 0.0) Zf = wsm / (0.4 * (Uf + 2 * wsm));
     if (s.ketdinh)
