@@ -705,6 +705,53 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
     return reduce_z_vals(m1, zr, p1, has_m1, has_p1);
 }
 
+// ---------------------------------------------------------------------------
+// Extraction (hydraulics).  Define: cells before `last` are written before
+// the end cell, whose continuity formula uses the new velocity at last-1.
+// ---------------------------------------------------------------------------
+// Cell p of line `line`; xat(rho) reads the line's solution row rho.
+template <int DIR, typename XAt>
+HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
+    const int L = DIR == 1 ? s.N : s.M;
+    int k, f, l;
+    if (!locate<DIR>(s, line, p, k, f, l)) return;
+    const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
+    const SegInfo &g = sc.g;
+    const int W = s.P;
+    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
+    double *__restrict__ tz = s.t_z;
+    double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
+    const int sn = g.sn;
+    const int rz = 2 * (p - 2), ru = rz + 1;
+    if (p < l) {
+        if (g.isB) {
+            tz[cell(p)] = xat(rz);
+            tv[cell(p)] = xat(ru);
+            if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : sc.ub_start;
+        } else {
+            tv[cell(p)] = xat(ru);
+            if (p > f) tz[cell(p)] = xat(rz);
+            if (p == f) {
+                const Pre e = preindex<DIR>(s, line, f);
+                tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * xat(ru)) / e.a2;
+            }
+        }
+    } else {  // p == l
+        const bool ext_Q = (DIR == 1 ? s.bienQ[3] : s.bienQ[0]) && l == L;
+        if (g.ran2) {
+            tv[cell(l)] = 0;
+            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
+        } else if (ext_Q) {
+            tv[cell(l)] = sc.ub_end;
+            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
+        } else {
+            const Pre e = preindex<DIR>(s, line, l);
+            const double prev = l > f ? xat(2 * (l - 1 - 2) + 1) : tv[cell(l - 1)];
+            tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
+        }
+    }
+}
+
 // Cyclic reduction of a whole line in LDS, one workgroup per line; thread
 // t owns rows q = t + k * blockDim.x, k < KPT.  The FP64 rows a thread needs again (its velocity row for the
 // recovery, its reduced z row for the residual) stay in its registers, so
@@ -837,9 +884,11 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 // same elimination (den, cp, v, w and the reduced system's multipliers).
 // Needs blockDim.x * K >= the number of unknowns.
 // ---------------------------------------------------------------------------
-template <typename T, bool REFINE, bool INTERLEAVED, int K>
-__global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows) {
+template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT>
+__global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows,
+                                              const DevState *__restrict__ S) {
     static_assert(K >= 2, "a chunk needs at least one interior unknown");
+    static_assert(!EXTRACT || INTERLEAVED, "extraction is that of a hydraulic row line");
     constexpr int R = INTERLEAVED ? 2 * K : K;  // rows per thread
     const SweepBufs B = Bin.scen(blockIdx.y);
     extern __shared__ double smem_part[];
@@ -1037,57 +1086,19 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         stage[e + e / R] = out[r];
     }
     __syncthreads();
+    if (EXTRACT) {
+        // VZ extraction (lines are rows, so the writes along j coalesce) of
+        // every cell of the row, from the solution staged in LDS: the
+        // solution never goes to HBM, and no extraction launch follows
+        const DevState &s = S[blockIdx.y];
+        for (int p = 2 + tid; p <= s.M; p += nt)
+            hyd_extract_cell<0>(s, line, p, [&](int rho) { return stage[rho + rho / R]; });
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < R; m++) {
         const int e = tid + m * nt;
         if (e < nrows) B.X[lo + e] = stage[e + e / R];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Extraction (hydraulics).  Define: cells before `last` are written before
-// the end cell, whose continuity formula uses the new velocity at last-1.
-// ---------------------------------------------------------------------------
-// Cell p of line `line`; xat(rho) reads the line's solution row rho.
-template <int DIR, typename XAt>
-HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
-    const int L = DIR == 1 ? s.N : s.M;
-    int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l)) return;
-    const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
-    const SegInfo &g = sc.g;
-    const int W = s.P;
-    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
-    double *__restrict__ tz = s.t_z;
-    double *__restrict__ tv = DIR == 1 ? s.t_u : s.t_v;
-    const int sn = g.sn;
-    const int rz = 2 * (p - 2), ru = rz + 1;
-    if (p < l) {
-        if (g.isB) {
-            tz[cell(p)] = xat(rz);
-            tv[cell(p)] = xat(ru);
-            if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : sc.ub_start;
-        } else {
-            tv[cell(p)] = xat(ru);
-            if (p > f) tz[cell(p)] = xat(rz);
-            if (p == f) {
-                const Pre e = preindex<DIR>(s, line, f);
-                tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * xat(ru)) / e.a2;
-            }
-        }
-    } else {  // p == l
-        const bool ext_Q = (DIR == 1 ? s.bienQ[3] : s.bienQ[0]) && l == L;
-        if (g.ran2) {
-            tv[cell(l)] = 0;
-            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
-        } else if (ext_Q) {
-            tv[cell(l)] = sc.ub_end;
-            if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
-        } else {
-            const Pre e = preindex<DIR>(s, line, l);
-            const double prev = l > f ? xat(2 * (l - 1 - 2) + 1) : tv[cell(l - 1)];
-            tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
-        }
     }
 }
 
@@ -1290,9 +1301,11 @@ static bool solver_shape(hd_ctx *ctx, int nz, int kind, int param, int &threads,
 }
 
 // Launches one fast-form line solver over `lines` lines of nscen scenarios.
+// extract (partition solver, VZ hydraulics): the solver also extracts t_z
+// and t_v from its LDS copy of the solution instead of writing X.
 template <typename T, bool REFINE, bool IL>
 static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int kind, int threads,
-                         int kpt, int start, int end, int nrows, const char *name) {
+                         int kpt, int start, int end, int nrows, const char *name, bool extract) {
     hipStream_t st = ctx->stream;
     const int lines = end - start + 1;
     int rc = HD_OK;
@@ -1312,11 +1325,17 @@ static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int
         auto go = [&](auto kern) {
             rc = allow_lds(ctx, kern, sm);
             if (rc) return;
-            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows)));
+            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows, S)));
         };
-        if (kpt == 2) go(k_part<T, REFINE, IL, 2>);
-        else if (kpt == 4) go(k_part<T, REFINE, IL, 4>);
-        else go(k_part<T, REFINE, IL, 8>);
+        if (IL && extract) {
+            if (kpt == 2) go(k_part<T, REFINE, IL, 2, IL>);
+            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, IL>);
+            else go(k_part<T, REFINE, IL, 8, IL>);
+        } else {
+            if (kpt == 2) go(k_part<T, REFINE, IL, 2, false>);
+            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, false>);
+            else go(k_part<T, REFINE, IL, 8, false>);
+        }
     }
     return rc;
 }
@@ -1332,18 +1351,31 @@ static const char *solver_name(int kind, bool f32, int dir, bool hyd) {
 }
 
 static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int dir, bool hyd, int kind,
-                      int param, int start, int end, int nrows) {
+                      int param, int start, int end, int nrows, bool extract = false) {
     const int nz = hyd ? nrows / 2 : nrows;
     if (kind == HD_SOLVER_DEFAULT) kind = HD_SOLVER_PARTITION;
     int threads, kpt;
     if (!solver_shape(ctx, nz, kind, param, threads, kpt)) return HD_E_UNSUPPORTED;
     const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
     const char *name = solver_name(kind, f32, dir, hyd);
+    extract = extract && hyd && dir == 0 && kind == HD_SOLVER_PARTITION;
     if (f32)
-        return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name)
-                   : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name);
-    return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name)
-               : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name);
+        return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
+                                                      extract)
+                   : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
+                                                       false);
+    return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
+                                                    extract)
+               : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
+                                                     false);
+}
+
+// Whether the VZ hydraulic sweep's solver extracts the solution itself
+// (the partition solver does; cyclic reduction leaves it to
+// k_hyd_extract).
+static bool solver_extracts(const hd_ctx *ctx) {
+    const int kind = ctx->solver_kind[0][1];
+    return kind == HD_SOLVER_DEFAULT || kind == HD_SOLVER_PARTITION;
 }
 
 template <int DIR, int LAYOUT>
@@ -1361,9 +1393,11 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
     } else {
         const int h = hyd ? 1 : 0;
+        const bool fused = DIR == 0 && hyd && solver_extracts(ctx);
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
-                        nrows);
+                        nrows, fused);
         if (rc) return rc;
+        if (fused) return HD_OK;  // t_z and t_v written by the solver
     }
     if (DIR == 1 && LAYOUT == LINE_MAJOR) {
         const dim3 xg((s.M + 3 + 63) / 64, (s.N - 1 + 7) / 8, n);
