@@ -120,8 +120,9 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
             param = int(pick.split("/")[1]) if pick and pick.startswith(kind + "/") else 0
             if kind == "part":
                 k = param or (4 if n <= 2048 else 8)
-                # the VZ hydraulic solve also extracts t_z, t_v (template EXTRACT)
-                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {'true' if what == 'vz' else 'false'}>"
+                # row-line solves (VZ, FS along i) also extract (template EXTRACT)
+                ex = "true" if what in ("vz", "fs_i") else "false"
+                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {ex}>"
             else:
                 threads = param or (512 if n > 1024 else 256)
                 kpt = -(-n // threads)

@@ -752,6 +752,36 @@ HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
     }
 }
 
+// FS extraction (sediment_transport.cpp:167-201, :316-349): negative values
+// become NDnen; the segment's end cells take the neighbouring value or keep
+// FS as the reference's end tests decide.  xv() reads the solution at p.
+template <int DIR, typename XV>
+HD_DEV void fs_extract_cell(const DevState &s, int line, int p, XV xv) {
+    int k, f, l;
+    if (!locate<DIR>(s, line, p, k, f, l) || l - f < 2) return;
+    if (p < f + 1 || p > l - 1) return;
+    bool ran1, ran2;
+    wall_flags(s, DIR, line, f, l, ran1, ran2);
+    const int W = s.P;
+    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
+    const double x = xv();
+    const double out = x < 0 ? s.c.NDnen : x;
+    const double *tvel = DIR == 1 ? s.t_u : s.t_v;
+    s.tFS[cell(p)] = out;
+    if (p == f + 1) {
+        if ((ran1) || (tvel[cell(f)] == 0))
+            s.tFS[cell(f)] = out;
+        else
+            s.tFS[cell(f)] = s.FS[cell(f)];
+    }
+    if (p == l - 1) {
+        if ((ran2) || (tvel[cell(l)] == 0))
+            s.tFS[cell(l)] = out;
+        else
+            s.tFS[cell(l)] = s.FS[cell(l)];
+    }
+}
+
 // Cyclic reduction of a whole line in LDS, one workgroup per line; thread
 // t owns rows q = t + k * blockDim.x, k < KPT.  The FP64 rows a thread needs again (its velocity row for the
 // recovery, its reduced z row for the residual) stay in its registers, so
@@ -888,7 +918,6 @@ template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT>
 __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows,
                                               const DevState *__restrict__ S) {
     static_assert(K >= 2, "a chunk needs at least one interior unknown");
-    static_assert(!EXTRACT || INTERLEAVED, "extraction is that of a hydraulic row line");
     constexpr int R = INTERLEAVED ? 2 * K : K;  // rows per thread
     const SweepBufs B = Bin.scen(blockIdx.y);
     extern __shared__ double smem_part[];
@@ -1087,12 +1116,17 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     }
     __syncthreads();
     if (EXTRACT) {
-        // VZ extraction (lines are rows, so the writes along j coalesce) of
-        // every cell of the row, from the solution staged in LDS: the
-        // solution never goes to HBM, and no extraction launch follows
+        // Extraction of a row line (the VZ sweep, or FS along i: writes
+        // along j coalesce) for every cell of the row, from the solution
+        // staged in LDS: the solution never goes to HBM, and no extraction
+        // launch follows
         const DevState &s = S[blockIdx.y];
-        for (int p = 2 + tid; p <= s.M; p += nt)
-            hyd_extract_cell<0>(s, line, p, [&](int rho) { return stage[rho + rho / R]; });
+        for (int p = 2 + tid; p <= s.M; p += nt) {
+            if constexpr (INTERLEAVED)
+                hyd_extract_cell<0>(s, line, p, [&](int rho) { return stage[rho + rho / R]; });
+            else
+                fs_extract_cell<0>(s, line, p, [&]() { return stage[(p - 2) + (p - 2) / R]; });
+        }
         return;
     }
 #pragma unroll
@@ -1116,36 +1150,6 @@ __global__ void k_hyd_extract(const DevState *__restrict__ S, SweepBufs Bin, int
     hyd_extract_cell<DIR>(s, line, p, [&](int rho) {
         return LAYOUT == ROW_MAJOR ? B.X[rm_idx(rho, line, ld)] : B.X[lm_idx(rho, line, ld)];
     });
-}
-
-// FS extraction (sediment_transport.cpp:167-201, :316-349): negative values
-// become NDnen; the segment's end cells take the neighbouring value or keep
-// FS as the reference's end tests decide.  xv() reads the solution at p.
-template <int DIR, typename XV>
-HD_DEV void fs_extract_cell(const DevState &s, int line, int p, XV xv) {
-    int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l) || l - f < 2) return;
-    if (p < f + 1 || p > l - 1) return;
-    bool ran1, ran2;
-    wall_flags(s, DIR, line, f, l, ran1, ran2);
-    const int W = s.P;
-    auto cell = [&](int q) { return DIR == 1 ? q * W + line : line * W + q; };
-    const double x = xv();
-    const double out = x < 0 ? s.c.NDnen : x;
-    const double *tvel = DIR == 1 ? s.t_u : s.t_v;
-    s.tFS[cell(p)] = out;
-    if (p == f + 1) {
-        if ((ran1) || (tvel[cell(f)] == 0))
-            s.tFS[cell(f)] = out;
-        else
-            s.tFS[cell(f)] = s.FS[cell(f)];
-    }
-    if (p == l - 1) {
-        if ((ran2) || (tvel[cell(l)] == 0))
-            s.tFS[cell(l)] = out;
-        else
-            s.tFS[cell(l)] = s.FS[cell(l)];
-    }
 }
 
 template <int DIR, int LAYOUT>
@@ -1301,8 +1305,9 @@ static bool solver_shape(hd_ctx *ctx, int nz, int kind, int param, int &threads,
 }
 
 // Launches one fast-form line solver over `lines` lines of nscen scenarios.
-// extract (partition solver, VZ hydraulics): the solver also extracts t_z
-// and t_v from its LDS copy of the solution instead of writing X.
+// extract (partition solver, row lines): the solver also extracts t_z and
+// t_v (hydraulics) or tFS (FS) from its LDS copy of the solution instead of
+// writing X.
 template <typename T, bool REFINE, bool IL>
 static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int kind, int threads,
                          int kpt, int start, int end, int nrows, const char *name, bool extract) {
@@ -1327,10 +1332,10 @@ static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int
             if (rc) return;
             LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows, S)));
         };
-        if (IL && extract) {
-            if (kpt == 2) go(k_part<T, REFINE, IL, 2, IL>);
-            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, IL>);
-            else go(k_part<T, REFINE, IL, 8, IL>);
+        if (extract) {
+            if (kpt == 2) go(k_part<T, REFINE, IL, 2, true>);
+            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true>);
+            else go(k_part<T, REFINE, IL, 8, true>);
         } else {
             if (kpt == 2) go(k_part<T, REFINE, IL, 2, false>);
             else if (kpt == 4) go(k_part<T, REFINE, IL, 4, false>);
@@ -1358,23 +1363,23 @@ static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int ns
     if (!solver_shape(ctx, nz, kind, param, threads, kpt)) return HD_E_UNSUPPORTED;
     const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
     const char *name = solver_name(kind, f32, dir, hyd);
-    extract = extract && hyd && dir == 0 && kind == HD_SOLVER_PARTITION;
+    extract = extract && dir == 0 && kind == HD_SOLVER_PARTITION;
     if (f32)
         return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
                                                       extract)
                    : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                       false);
+                                                       extract);
     return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
                                                     extract)
                : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                     false);
+                                                     extract);
 }
 
-// Whether the VZ hydraulic sweep's solver extracts the solution itself
-// (the partition solver does; cyclic reduction leaves it to
-// k_hyd_extract).
-static bool solver_extracts(const hd_ctx *ctx) {
-    const int kind = ctx->solver_kind[0][1];
+// Whether the solver of a row-line sweep (VZ, or FS along i) extracts the
+// solution itself (the partition solver does; cyclic reduction leaves it
+// to k_hyd_extract / k_fs_extract).
+static bool solver_extracts(const hd_ctx *ctx, bool hyd) {
+    const int kind = ctx->solver_kind[0][hyd ? 1 : 0];
     return kind == HD_SOLVER_DEFAULT || kind == HD_SOLVER_PARTITION;
 }
 
@@ -1393,11 +1398,11 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
                k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
     } else {
         const int h = hyd ? 1 : 0;
-        const bool fused = DIR == 0 && hyd && solver_extracts(ctx);
+        const bool fused = DIR == 0 && solver_extracts(ctx, hyd);
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
                         nrows, fused);
         if (rc) return rc;
-        if (fused) return HD_OK;  // t_z and t_v written by the solver
+        if (fused) return HD_OK;  // t_z, t_v (or tFS) written by the solver
     }
     if (DIR == 1 && LAYOUT == LINE_MAJOR) {
         const dim3 xg((s.M + 3 + 63) / 64, (s.N - 1 + 7) / 8, n);

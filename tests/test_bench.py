@@ -14,6 +14,9 @@ def test_solver_stage_symbols_follow_the_tuned_choice():
     assert bench.kernel_symbol("k_part<f64,uz>", SC, chosen, 1) == "k_part<double, false, true, 4, false>"
     assert bench.kernel_symbol("k_part<f64,vz>", SC, chosen, 1) == "k_part<double, false, true, 8, true>"
     assert bench.kernel_symbol("k_part<f32r,vz>", SC, chosen, 1) == "k_part<float, true, true, 8, true>"
+    # row-line solves extract their solution themselves (EXTRACT = true)
+    assert bench.kernel_symbol("k_part<f64,fs_i>", SC, ["part/4", "part/8", "part/8", "part/2"], 1) == \
+        "k_part<double, false, false, 2, true>"
     # cyclic reduction: 1023 unknowns over 256 threads -> 4 per thread
     assert bench.kernel_symbol("k_cr_reg<f64,fs_j>", SC, chosen, 1) == "k_cr_reg<double, false, false, 4>"
     # default workgroup rule (512 above 1024 unknowns): 2047 / 512 -> 4
