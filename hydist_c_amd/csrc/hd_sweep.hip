@@ -914,10 +914,11 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 // same elimination (den, cp, v, w and the reduced system's multipliers).
 // Needs blockDim.x * K >= the number of unknowns.
 // ---------------------------------------------------------------------------
-template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT>
+template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT, bool ASSEMBLE = false>
 __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows,
                                               const DevState *__restrict__ S) {
     static_assert(K >= 2, "a chunk needs at least one interior unknown");
+    static_assert(!ASSEMBLE || (INTERLEAVED && EXTRACT), "assembly is that of a hydraulic row line");
     constexpr int R = INTERLEAVED ? 2 * K : K;  // rows per thread
     const SweepBufs B = Bin.scen(blockIdx.y);
     extern __shared__ double smem_part[];
@@ -946,22 +947,47 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     // compiler wait for each array's loads alone.
     double own[4][R], prv[4];
     double ld4[4][R];
+    if constexpr (ASSEMBLE) {
+        // VZ rows assembled here (as k_hyd_assemble<0, 1>): position
+        // q = tid + m * nt gives rows 2q (z) and 2q + 1 (velocity), kept as
+        // ld4[.][2m] and ld4[.][2m + 1]; positions past the line give
+        // identity rows
+        const DevState &s = S[blockIdx.y];
+        const double f4 = 2 * s.c.g * s.c.dTchia2dY;
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
-        const double *src = a == 0 ? B.A : (a == 1 ? B.B : (a == 2 ? B.C : B.D));
-        const double fill = a == 1 ? 1.0 : 0.0;
+        for (int m = 0; m < K; m++) {
+            const int p = tid + m * nt + 2;
+            Row rz = identity_row(), ru = identity_row();
+            int k, f, l;
+            if (p <= s.M && locate<0>(s, line, p, k, f, l)) {
+                const SegCtx sc = seg_ctx<0>(s, line, f, l);
+                hyd_rows<0>(s, line, p, sc, f4, rz, ru);
+                if (p == f) {
+                    B.sys0[(size_t)line * s.capH + k] = sc.rho0;
+                    B.sysn[(size_t)line * s.capH + k] = sc.g.sn;
+                }
+            }
+            ld4[0][2 * m] = rz.a; ld4[1][2 * m] = rz.b; ld4[2][2 * m] = rz.c; ld4[3][2 * m] = rz.d;
+            ld4[0][2 * m + 1] = ru.a; ld4[1][2 * m + 1] = ru.b; ld4[2][2 * m + 1] = ru.c; ld4[3][2 * m + 1] = ru.d;
+        }
+    } else {
 #pragma unroll
-        for (int m = 0; m < R; m++) {
-            const int e = tid + m * nt;
-            const double v = src[lo + min(e, nrows - 1)];
-            ld4[a][m] = e < nrows ? v : fill;
+        for (int a = 0; a < 4; a++) {
+            const double *src = a == 0 ? B.A : (a == 1 ? B.B : (a == 2 ? B.C : B.D));
+            const double fill = a == 1 ? 1.0 : 0.0;
+#pragma unroll
+            for (int m = 0; m < R; m++) {
+                const int e = tid + m * nt;
+                const double v = src[lo + min(e, nrows - 1)];
+                ld4[a][m] = e < nrows ? v : fill;
+            }
         }
     }
 #pragma unroll
     for (int a = 0; a < 4; a++) {
 #pragma unroll
         for (int m = 0; m < R; m++) {
-            const int e = tid + m * nt;
+            const int e = ASSEMBLE ? 2 * (tid + (m / 2) * nt) + (m & 1) : tid + m * nt;
             stage[e + e / R] = ld4[a][m];
         }
         __syncthreads();
@@ -1310,7 +1336,7 @@ static bool solver_shape(hd_ctx *ctx, int nz, int kind, int param, int &threads,
 // writing X.
 template <typename T, bool REFINE, bool IL>
 static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int kind, int threads,
-                         int kpt, int start, int end, int nrows, const char *name, bool extract) {
+                         int kpt, int start, int end, int nrows, const char *name, bool extract, bool assemble) {
     hipStream_t st = ctx->stream;
     const int lines = end - start + 1;
     int rc = HD_OK;
@@ -1332,7 +1358,11 @@ static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int
             if (rc) return;
             LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows, S)));
         };
-        if (extract) {
+        if (IL && extract && assemble) {
+            if (kpt == 2) go(k_part<T, REFINE, IL, 2, true, IL>);
+            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true, IL>);
+            else go(k_part<T, REFINE, IL, 8, true, IL>);
+        } else if (extract) {
             if (kpt == 2) go(k_part<T, REFINE, IL, 2, true>);
             else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true>);
             else go(k_part<T, REFINE, IL, 8, true>);
@@ -1356,7 +1386,7 @@ static const char *solver_name(int kind, bool f32, int dir, bool hyd) {
 }
 
 static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int dir, bool hyd, int kind,
-                      int param, int start, int end, int nrows, bool extract = false) {
+                      int param, int start, int end, int nrows, bool extract = false, bool assemble = false) {
     const int nz = hyd ? nrows / 2 : nrows;
     if (kind == HD_SOLVER_DEFAULT) kind = HD_SOLVER_PARTITION;
     int threads, kpt;
@@ -1364,15 +1394,16 @@ static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int ns
     const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
     const char *name = solver_name(kind, f32, dir, hyd);
     extract = extract && dir == 0 && kind == HD_SOLVER_PARTITION;
+    assemble = assemble && extract && hyd;
     if (f32)
         return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                      extract)
+                                                      extract, assemble)
                    : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                       extract);
+                                                       extract, false);
     return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                    extract)
+                                                    extract, assemble)
                : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                     extract);
+                                                     extract, false);
 }
 
 // Whether the solver of a row-line sweep (VZ, or FS along i) extracts the
@@ -1390,7 +1421,10 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = LAYOUT == ROW_MAJOR ? B.ld_rm[DIR] : B.ld_lm[DIR];
-    int rc = launch_assembly<DIR, LAYOUT>(ctx, S, s, B, n, hyd, start, end, t, s_start, ctx->capturing ? 1 : 0);
+    // the VZ rows are assembled inside the partition solver
+    const bool fused_asm = LAYOUT == LINE_MAJOR && DIR == 0 && hyd && solver_extracts(ctx, true);
+    int rc = HD_OK;
+    if (!fused_asm) rc = launch_assembly<DIR, LAYOUT>(ctx, S, s, B, n, hyd, start, end, t, s_start, ctx->capturing ? 1 : 0);
     if (rc) return rc;
     const int lines = end - start + 1;
     if (LAYOUT == ROW_MAJOR) {
@@ -1400,7 +1434,7 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         const int h = hyd ? 1 : 0;
         const bool fused = DIR == 0 && solver_extracts(ctx, hyd);
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
-                        nrows, fused);
+                        nrows, fused, fused_asm);
         if (rc) return rc;
         if (fused) return HD_OK;  // t_z, t_v (or tFS) written by the solver
     }
