@@ -55,6 +55,11 @@ def stage_bytes(name, sim, sc):
         uz = any(t in name for t in ("uz", "fs_j", "<j>"))
         n = (lines_uz * pos_uz) if uz else (lines_vz * pos_vz)
         hyd = "fs" not in name
+        if name.startswith("k_part<") and name.endswith(",vz>"):
+            # the VZ sweep in one kernel (rows assembled, solved and
+            # extracted on chip): the assembly's 12 state/static fields in,
+            # t_z and t_v out
+            return (12 * 8 + 2 * 8) * float(n)
         if name.startswith(("k_cr", "k_part", "k_thomas")):
             # tridiagonal solve: 4 coefficients in + solution out per unknown
             return (80.0 if hyd else 40.0) * n
@@ -120,9 +125,11 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
             param = int(pick.split("/")[1]) if pick and pick.startswith(kind + "/") else 0
             if kind == "part":
                 k = param or (4 if n <= 2048 else 8)
-                # row-line solves (VZ, FS along i) also extract (template EXTRACT)
+                # row-line solves (VZ, FS along i) also extract (template
+                # EXTRACT); the VZ one also assembles its rows (ASSEMBLE)
                 ex = "true" if what in ("vz", "fs_i") else "false"
-                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {ex}>"
+                asm = "true" if what == "vz" else "false"
+                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {ex}, {asm}>"
             else:
                 threads = param or (512 if n > 1024 else 256)
                 kpt = -(-n // threads)
@@ -418,9 +425,9 @@ def main():
         if traffic is not None:
             roofline["traffic_source"] = "profiles/c2_2048x1024_fast_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     # the tridiagonal solve kernel of the best form, at its own roofline.  The
-    # partition solver of the VZ sweep also extracts t_z and t_v (no
-    # extraction launch follows it), so the pure solve reported is the
-    # slowest other hydraulic one; the fused kernel is reported beside it.
+    # partition solver of the VZ sweep also assembles its rows and extracts
+    # t_z and t_v (the whole sweep is one kernel), so the pure solve reported
+    # is the slowest other hydraulic one; the fused kernel is reported beside it.
     tri = None
     solver = [(k, v) for k, v in stats.items()
               if k.startswith(("k_cr", "k_part", "k_thomas")) and "fs" not in k and v[0] > 0 and v[1] > 0]
@@ -439,7 +446,8 @@ def main():
     if fused:
         k, (launches, total_ms) = fused[0]
         f = solver_roof(k, launches, total_ms)
-        f["note"] = "solve + extraction of t_z, t_v in one kernel; bytes = the solve's (rows in, z and velocity out)"
+        f["note"] = ("the VZ sweep in one kernel: rows assembled from the state, solved and t_z, t_v extracted "
+                     "on chip; bytes = the 12 state/static fields in + t_z, t_v out")
         if tri is None:
             tri = f
         else:
