@@ -60,6 +60,9 @@ def stage_bytes(name, sim, sc):
             # extracted on chip): the assembly's 12 state/static fields in,
             # t_z and t_v out
             return (12 * 8 + 2 * 8) * float(n)
+        if name.startswith("k_part<") and name.endswith(",fs_i>"):
+            # the FS sweep along i in one kernel: 12 fields in, tFS out
+            return (12 * 8 + 8) * float(n)
         if name.startswith(("k_cr", "k_part", "k_thomas")):
             # tridiagonal solve: 4 coefficients in + solution out per unknown
             return (80.0 if hyd else 40.0) * n
@@ -125,11 +128,10 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
             param = int(pick.split("/")[1]) if pick and pick.startswith(kind + "/") else 0
             if kind == "part":
                 k = param or (4 if n <= 2048 else 8)
-                # row-line solves (VZ, FS along i) also extract (template
-                # EXTRACT); the VZ one also assembles its rows (ASSEMBLE)
+                # row-line sweeps (VZ, FS along i) are one kernel: the solver
+                # assembles its rows and extracts (EXTRACT, ASSEMBLE)
                 ex = "true" if what in ("vz", "fs_i") else "false"
-                asm = "true" if what == "vz" else "false"
-                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {ex}, {asm}>"
+                return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}, {ex}, {ex}>"
             else:
                 threads = param or (512 if n > 1024 else 256)
                 kpt = -(-n // threads)

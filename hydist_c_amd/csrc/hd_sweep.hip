@@ -916,9 +916,10 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 // ---------------------------------------------------------------------------
 template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT, bool ASSEMBLE = false>
 __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end, int nrows,
-                                              const DevState *__restrict__ S) {
+                                              const DevState *__restrict__ S, double t_in, double s_start,
+                                              int from_clock) {
     static_assert(K >= 2, "a chunk needs at least one interior unknown");
-    static_assert(!ASSEMBLE || (INTERLEAVED && EXTRACT), "assembly is that of a hydraulic row line");
+    static_assert(!ASSEMBLE || EXTRACT, "assembly is that of a row-line sweep, which extracts too");
     constexpr int R = INTERLEAVED ? 2 * K : K;  // rows per thread
     const SweepBufs B = Bin.scen(blockIdx.y);
     extern __shared__ double smem_part[];
@@ -947,7 +948,37 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     // compiler wait for each array's loads alone.
     double own[4][R], prv[4];
     double ld4[4][R];
-    if constexpr (ASSEMBLE) {
+    if constexpr (ASSEMBLE && !INTERLEAVED) {
+        // FS rows along i assembled here (as k_fs_assemble<0, 1>): position
+        // p = tid + m * nt + 2 gives row p - 2, kept as ld4[.][m]
+        const DevState &s = S[blockIdx.y];
+        const double t = from_clock ? s.clock[0] : t_in;
+#pragma unroll
+        for (int m = 0; m < K; m++) {
+            const int p = tid + m * nt + 2;
+            Row r = identity_row();
+            int k = -1, f = 0, l = 0;
+            if (p <= s.M) {
+                if (locate<0>(s, line, p, k, f, l) && l - f >= 2) {
+                    bool ran1, ran2;
+                    wall_flags(s, 0, line, f, l, ran1, ran2);
+                    if (p >= f + 1 && p <= l - 1) {
+                        fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                        if (p == f + 1) r.a = 0.0;
+                        if (p == l - 1) r.c = 0.0;
+                    }
+                    if (p == f) {
+                        B.sys0[(size_t)line * s.capH + k] = f + 1 - 2;
+                        B.sysn[(size_t)line * s.capH + k] = l - f - 2;
+                    }
+                } else if (k >= 0 && p == f) {
+                    B.sys0[(size_t)line * s.capH + k] = -1;
+                    B.sysn[(size_t)line * s.capH + k] = -1;
+                }
+            }
+            ld4[0][m] = r.a; ld4[1][m] = r.b; ld4[2][m] = r.c; ld4[3][m] = r.d;
+        }
+    } else if constexpr (ASSEMBLE) {
         // VZ rows assembled here (as k_hyd_assemble<0, 1>): position
         // q = tid + m * nt gives rows 2q (z) and 2q + 1 (velocity), kept as
         // ld4[.][2m] and ld4[.][2m + 1]; positions past the line give
@@ -987,7 +1018,7 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     for (int a = 0; a < 4; a++) {
 #pragma unroll
         for (int m = 0; m < R; m++) {
-            const int e = ASSEMBLE ? 2 * (tid + (m / 2) * nt) + (m & 1) : tid + m * nt;
+            const int e = ASSEMBLE && INTERLEAVED ? 2 * (tid + (m / 2) * nt) + (m & 1) : tid + m * nt;
             stage[e + e / R] = ld4[a][m];
         }
         __syncthreads();
@@ -1336,7 +1367,9 @@ static bool solver_shape(hd_ctx *ctx, int nz, int kind, int param, int &threads,
 // writing X.
 template <typename T, bool REFINE, bool IL>
 static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int kind, int threads,
-                         int kpt, int start, int end, int nrows, const char *name, bool extract, bool assemble) {
+                         int kpt, int start, int end, int nrows, const char *name, bool extract, bool assemble,
+                         double t, double s_start) {
+    const int from_clock = ctx->capturing ? 1 : 0;
     hipStream_t st = ctx->stream;
     const int lines = end - start + 1;
     int rc = HD_OK;
@@ -1356,12 +1389,14 @@ static int launch_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int
         auto go = [&](auto kern) {
             rc = allow_lds(ctx, kern, sm);
             if (rc) return;
-            LAUNCH_RC(ctx, rc, name, (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows, S)));
+            LAUNCH_RC(ctx, rc, name,
+                      (kern<<<dim3(lines, nscen), threads, sm, st>>>(B, start, end, nrows, S, t, s_start,
+                                                                      from_clock)));
         };
-        if (IL && extract && assemble) {
-            if (kpt == 2) go(k_part<T, REFINE, IL, 2, true, IL>);
-            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true, IL>);
-            else go(k_part<T, REFINE, IL, 8, true, IL>);
+        if (extract && assemble) {
+            if (kpt == 2) go(k_part<T, REFINE, IL, 2, true, true>);
+            else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true, true>);
+            else go(k_part<T, REFINE, IL, 8, true, true>);
         } else if (extract) {
             if (kpt == 2) go(k_part<T, REFINE, IL, 2, true>);
             else if (kpt == 4) go(k_part<T, REFINE, IL, 4, true>);
@@ -1386,7 +1421,8 @@ static const char *solver_name(int kind, bool f32, int dir, bool hyd) {
 }
 
 static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int dir, bool hyd, int kind,
-                      int param, int start, int end, int nrows, bool extract = false, bool assemble = false) {
+                      int param, int start, int end, int nrows, bool extract = false, bool assemble = false,
+                      double t = 0.0, double s_start = 0.0) {
     const int nz = hyd ? nrows / 2 : nrows;
     if (kind == HD_SOLVER_DEFAULT) kind = HD_SOLVER_PARTITION;
     int threads, kpt;
@@ -1394,16 +1430,16 @@ static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int ns
     const bool f32 = ctx->cfg.form == HD_FORM_FP32_REFINE;
     const char *name = solver_name(kind, f32, dir, hyd);
     extract = extract && dir == 0 && kind == HD_SOLVER_PARTITION;
-    assemble = assemble && extract && hyd;
+    assemble = assemble && extract;
     if (f32)
         return hyd ? launch_solver<float, true, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                      extract, assemble)
+                                                      extract, assemble, t, s_start)
                    : launch_solver<float, true, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                       extract, false);
+                                                       extract, assemble, t, s_start);
     return hyd ? launch_solver<double, false, true>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                    extract, assemble)
+                                                    extract, assemble, t, s_start)
                : launch_solver<double, false, false>(ctx, S, B, nscen, kind, threads, kpt, start, end, nrows, name,
-                                                     extract, false);
+                                                     extract, assemble, t, s_start);
 }
 
 // Whether the solver of a row-line sweep (VZ, or FS along i) extracts the
@@ -1421,8 +1457,9 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = LAYOUT == ROW_MAJOR ? B.ld_rm[DIR] : B.ld_lm[DIR];
-    // the VZ rows are assembled inside the partition solver
-    const bool fused_asm = LAYOUT == LINE_MAJOR && DIR == 0 && hyd && solver_extracts(ctx, true);
+    // row-line sweeps (VZ, FS along i) are assembled inside the partition
+    // solver
+    const bool fused_asm = LAYOUT == LINE_MAJOR && DIR == 0 && solver_extracts(ctx, hyd);
     int rc = HD_OK;
     if (!fused_asm) rc = launch_assembly<DIR, LAYOUT>(ctx, S, s, B, n, hyd, start, end, t, s_start, ctx->capturing ? 1 : 0);
     if (rc) return rc;
@@ -1434,7 +1471,7 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
         const int h = hyd ? 1 : 0;
         const bool fused = DIR == 0 && solver_extracts(ctx, hyd);
         rc = run_solver(ctx, S, B, n, DIR, hyd, ctx->solver_kind[DIR][h], ctx->solver_param[DIR][h], start, end,
-                        nrows, fused, fused_asm);
+                        nrows, fused, fused_asm, t, s_start);
         if (rc) return rc;
         if (fused) return HD_OK;  // t_z, t_v (or tFS) written by the solver
     }

@@ -16,7 +16,7 @@ def test_solver_stage_symbols_follow_the_tuned_choice():
     assert bench.kernel_symbol("k_part<f32r,vz>", SC, chosen, 1) == "k_part<float, true, true, 8, true, true>"
     # row-line solves extract their solution themselves (EXTRACT = true)
     assert bench.kernel_symbol("k_part<f64,fs_i>", SC, ["part/4", "part/8", "part/8", "part/2"], 1) == \
-        "k_part<double, false, false, 2, true, false>"
+        "k_part<double, false, false, 2, true, true>"
     # cyclic reduction: 1023 unknowns over 256 threads -> 4 per thread
     assert bench.kernel_symbol("k_cr_reg<f64,fs_j>", SC, chosen, 1) == "k_cr_reg<double, false, false, 4>"
     # default workgroup rule (512 above 1024 unknowns): 2047 / 512 -> 4
@@ -43,7 +43,9 @@ def test_stage_bytes():
     assert bench.stage_bytes("k_cr_reg<f32r,vz>", None, SC) == 80.0 * pos
     # the partition solver's VZ sweep assembles and extracts too: 12 fields in, t_z and t_v out
     assert bench.stage_bytes("k_part<f64,vz>", None, SC) == 112.0 * pos
-    assert bench.stage_bytes("k_part<f64,fs_i>", None, SC) == 40.0 * pos
+    assert bench.stage_bytes("k_part<f64,fs_j>", None, SC) == 40.0 * pos
+    # the FS sweep along i in one kernel: 12 fields in, tFS out
+    assert bench.stage_bytes("k_part<f64,fs_i>", None, SC) == 104.0 * pos
     assert bench.stage_bytes("k_hyd_assemble<uz>", None, SC) == 160.0 * pos
     cells = (SC.M + 3) * (SC.N + 3)
     assert bench.stage_bytes("k_segment", None, SC) == 8.0 * cells
