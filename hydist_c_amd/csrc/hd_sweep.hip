@@ -1494,9 +1494,32 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     return HD_OK;
 }
 
+// Copies the fields a row-line sweep writes (t_z and t_v, or tFS) of every
+// scenario into the Thomas scratch (unused by the fast forms), or back.
+// The caller checks that a field fits in one scenario's scratch.
+static int swap_sweep_outputs(hd_ctx *ctx, int n, bool hyd, bool save) {
+    const SweepBufs &B = ctx->sweep;
+    for (int k = 0; k < n; k++) {
+        const DevState &sk = k == 0 ? ctx->hs : ctx->hss[k - 1];
+        const size_t bytes = (size_t)sk.P * sk.H * sizeof(double);
+        double *f0 = hyd ? sk.t_z : sk.tFS, *f1 = sk.t_v;
+        double *a = B.Ap + (size_t)k * B.stride, *b = B.Bp + (size_t)k * B.stride;
+        if (hipMemcpyAsync(save ? a : f0, save ? f0 : a, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
+            return HD_E_HIP;
+        if (hyd && hipMemcpyAsync(save ? b : f1, save ? f1 : b, bytes, hipMemcpyDeviceToDevice, ctx->stream) !=
+                       hipSuccess)
+            return HD_E_HIP;
+    }
+    return HD_OK;
+}
+
 // Times every line-solver candidate on one sweep and keeps the fastest.
-// The system is assembled once from the current state; only sweep scratch
-// is written, so the simulation state is untouched.
+// Column-line sweeps: the system is assembled once from the current state
+// and only the solve is timed (assembly and extraction are the same for
+// every candidate).  Row-line sweeps: the partition solver assembles and
+// extracts inside its kernel, so each candidate is timed as the whole
+// sweep, with the fields the sweep writes saved before and put back after.
+// Either way only sweep scratch keeps a trace: the state is untouched.
 template <int DIR>
 static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
                      double t, double s_start, std::string &report) {
@@ -1504,8 +1527,20 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = B.ld_lm[DIR];
-    int arc = launch_assembly<DIR, LINE_MAJOR>(ctx, S, s, B, n, hyd, start, end, t, s_start, 0);
+    // (a field fits the scratch on all but the smallest meshes, where the
+    // solve alone is timed)
+    const bool whole = DIR == 0 && (size_t)s.P * s.H <= B.stride;
+    int arc = whole ? swap_sweep_outputs(ctx, n, hyd, true)
+                    : launch_assembly<DIR, LINE_MAJOR>(ctx, S, s, B, n, hyd, start, end, t, s_start, 0);
     if (arc) return arc;
+    const int h = hyd ? 1 : 0;
+    const int kind0 = ctx->solver_kind[DIR][h], param0 = ctx->solver_param[DIR][h];
+    auto candidate = [&](int kind, int param) {
+        if (!whole) return run_solver(ctx, S, B, n, DIR, hyd, kind, param, start, end, nrows);
+        ctx->solver_kind[DIR][h] = kind;
+        ctx->solver_param[DIR][h] = param;
+        return sweep_impl<DIR, LINE_MAJOR>(ctx, S, s, n, hyd, start, end, t, s_start);
+    };
     const int nz = hyd ? nrows / 2 : nrows;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return HD_E_HIP;
@@ -1523,9 +1558,9 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     for (const auto &c : cands) {
         int threads, kpt;
         if (!solver_shape(ctx, nz, c[0], c[1], threads, kpt)) continue;  // too long for this candidate
-        rc = run_solver(ctx, S, B, n, DIR, hyd, c[0], c[1], start, end, nrows);  // warm-up
+        rc = candidate(c[0], c[1]);  // warm-up
         if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? HD_OK : HD_E_HIP;
-        for (int r = 0; r < 3 && !rc; r++) rc = run_solver(ctx, S, B, n, DIR, hyd, c[0], c[1], start, end, nrows);
+        for (int r = 0; r < 3 && !rc; r++) rc = candidate(c[0], c[1]);
         if (!rc) rc = hipEventRecord(e1, st) == hipSuccess ? HD_OK : HD_E_HIP;
         if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? HD_OK : HD_E_HIP;
         if (rc) break;
@@ -1544,6 +1579,13 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     ctx->prof = prof;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    ctx->solver_kind[DIR][h] = kind0;
+    ctx->solver_param[DIR][h] = param0;
+    if (whole) {
+        const int src = swap_sweep_outputs(ctx, n, hyd, false);
+        if (!rc) rc = src;
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = HD_E_HIP;
+    }
     if (rc) return rc;
     if (!best_kind) return HD_E_UNSUPPORTED;  // ctx->err names the line length
     ctx->solver_kind[DIR][hyd ? 1 : 0] = best_kind;
