@@ -11,6 +11,10 @@
 //   extract:  one thread per cell writes the solution back into t_z and
 //             t_u / t_v (UVZSolver_multithread.cpp:472-603) or tFS
 //             (sediment_transport.cpp:167-201, :316-349).
+// In the fast forms with the partition solver, a sweep whose lines are
+// rows (VZ, FS along i) is one kernel: k_part assembles the rows itself
+// and extracts from its LDS copy of the solution, so neither the rows nor
+// the solution go through HBM.
 //
 // Row space of a line: position p in [2, L] (L = N along columns, M along
 // rows) owns row rho = 2(p-2) for z_p and rho = 2(p-2)+1 for the velocity
@@ -912,6 +916,11 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 // while each thread works on a contiguous chunk.  With REFINE the T
 // solution is corrected once by solving for the FP64 residual with the
 // same elimination (den, cp, v, w and the reduced system's multipliers).
+// Row lines (the VZ sweep, FS along i; S, the time and the clock flag are
+// used only then): with ASSEMBLE the rows are formed from the state inside
+// the kernel instead of read from B.A..D, and with EXTRACT the results
+// (t_z and t_v, or tFS) are extracted from the LDS copy of the solution
+// instead of X being written, so the whole sweep is this one kernel.
 // Needs blockDim.x * K >= the number of unknowns.
 // ---------------------------------------------------------------------------
 template <typename T, bool REFINE, bool INTERLEAVED, int K, bool EXTRACT, bool ASSEMBLE = false>
