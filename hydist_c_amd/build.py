@@ -52,21 +52,6 @@ def build_compat_driver(verbose=False):
         print("built", exe)
     return exe
 
-def build_copy_calibrate(verbose=False):
-    """hydist_c_amd/_build/copy_calibrate: a copy of known size whose HBM
-    counters scale the library kernels' FETCH_SIZE / WRITE_SIZE."""
-    src = os.path.join(PKG, "tools", "copy_calibrate.hip")
-    exe = os.path.join(OBJ, "copy_calibrate")
-    os.makedirs(OBJ, exist_ok=True)
-    cmd = [HIPCC] + [f for f in FLAGS if f != "-fPIC"] + [src, "-o", exe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"copy_calibrate build failed:\n{r.stdout}\n{r.stderr}")
-    if verbose:
-        print("built", exe)
-    return exe
-
-
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -81,7 +66,6 @@ def build(verbose=False):
     if verbose:
         print("built", LIB)
     build_compat_driver(verbose)
-    build_copy_calibrate(verbose)
     return LIB
 
 

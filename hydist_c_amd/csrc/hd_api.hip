@@ -131,6 +131,14 @@ int check_launch(hd_ctx *ctx, const char *what) {
     return HD_OK;
 }
 
+// Whole-step graphs hold the launch shapes and kernels chosen when they were
+// captured; every setting that changes either drops them, so the next
+// graph step captures again.
+void drop_graphs(hd_ctx *ctx) {
+    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
 static hipEvent_t pool_get(hd_ctx *ctx) {
     if (!ctx->event_pool.empty()) {
         hipEvent_t e = ctx->event_pool.back();
@@ -516,18 +524,57 @@ static int step_graph(hd_ctx *ctx, double *t) {
     return HD_OK;
 }
 
+// Reads the watchdog's flag (the stream must be idle): HD_E_NUMERIC once a
+// check has found a non-finite value.
+static int watchdog_status(hd_ctx *ctx) {
+    if (!ctx->wd_tripped && ctx->wd_every > 0) {
+        int flag = 0;
+        HD_HIP(ctx, hipMemcpy(&flag, ctx->wd_flag, sizeof(int), hipMemcpyDeviceToHost));
+        ctx->wd_tripped = flag != 0;
+    }
+    if (ctx->wd_tripped)
+        return set_err(ctx, HD_E_NUMERIC, "watchdog: non-finite z, u or v in a wet cell (checked every " +
+                                              std::to_string(ctx->wd_every) + " steps)");
+    return HD_OK;
+}
+
+// One full step, then the watchdog's check when one is due; *checked tells
+// whether a check was queued.
+static int step_checked(hd_ctx *ctx, double *t, bool *checked) {
+    *checked = false;
+    int rc = (ctx->use_graph && !ctx->fault_stage) ? step_graph(ctx, t) : step_eager(ctx, t);
+    if (rc || ctx->wd_every <= 0) return rc;
+    if (++ctx->wd_steps % ctx->wd_every == 0) {
+        Launch2D g = grid2d(ctx->hs, ctx->nscen);
+        LAUNCH(ctx, "k_watchdog", k_watchdog<<<g.grid, g.block, 0, ctx->stream>>>(ctx->ds, ctx->wd_flag));
+        *checked = true;
+    }
+    return HD_OK;
+}
+
 int hd_step(hd_ctx *ctx, double *t) {
     if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
     if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
-    if (ctx->use_graph && !ctx->fault_stage) return step_graph(ctx, t);
-    return step_eager(ctx, t);
+    bool checked;
+    return step_checked(ctx, t, &checked);
+}
+
+// A step of hd_run / hd_run_cb: waits for the watchdog's check when one ran.
+static int run_step(hd_ctx *ctx, double *t) {
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
+    bool checked;
+    int rc = step_checked(ctx, t, &checked);
+    if (rc || !checked) return rc;
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return watchdog_status(ctx);
 }
 
 int hd_run(hd_ctx *ctx, double *t, double Tmax) {
     if (!ctx || !t) return set_err(ctx, HD_E_INVALID, "ctx or t is NULL");
+    if (ctx->wd_tripped) return watchdog_status(ctx);
     const int T = (int)Tmax;
     while (*t < T) {
-        int rc = hd_step(ctx, t);
+        int rc = run_step(ctx, t);
         if (rc) return rc;
     }
     return HD_OK;
@@ -535,15 +582,34 @@ int hd_run(hd_ctx *ctx, double *t, double Tmax) {
 
 int hd_run_cb(hd_ctx *ctx, double *t, double Tmax, int interval, hd_output_cb cb, void *user) {
     if (!ctx || !t || interval < 1) return set_err(ctx, HD_E_INVALID, "ctx/t NULL or interval < 1");
+    if (ctx->wd_tripped) return watchdog_status(ctx);
     const int T = (int)Tmax;
     while (*t < T) {
-        int rc = hd_step(ctx, t);
+        int rc = run_step(ctx, t);
         if (rc) return rc;
         if (cb && ((int)*t % interval == 0) && (*t - (int)*t == 0)) {
             HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
             if (cb(ctx, *t, user)) return set_err(ctx, HD_E_CANCELLED, "stopped by the output callback");
         }
     }
+    return HD_OK;
+}
+
+int hd_set_watchdog(hd_ctx *ctx, int every) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (every < 0) return set_err(ctx, HD_E_INVALID, "watchdog interval must be >= 0");
+    LAUNCH(ctx, "k_set_flag", k_set_flag<<<1, 64, 0, ctx->stream>>>(ctx->wd_flag, 0));
+    HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->wd_every = every;
+    ctx->wd_steps = 0;
+    ctx->wd_tripped = false;
+    return HD_OK;
+}
+
+int hd_inject_fault(hd_ctx *ctx, int stage) {
+    if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    if (stage < 0 || stage > 2) return set_err(ctx, HD_E_INVALID, "fault stage must be 0 (off), 1 (UZ) or 2 (VZ)");
+    ctx->fault_stage = stage;
     return HD_OK;
 }
 
@@ -592,7 +658,7 @@ int hd_bedload(hd_ctx *ctx, double t) {
 int hd_sync(hd_ctx *ctx) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return HD_OK;
+    return watchdog_status(ctx);
 }
 
 int hd_copy_out(hd_ctx *ctx, int field, double *dst, size_t n) {
@@ -727,7 +793,7 @@ int hd_destroy(hd_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     prof_resolve(ctx);
-    for (auto &kv : ctx->graphs) (void)hipGraphExecDestroy(kv.second);
+    drop_graphs(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     for (void *p : ctx->allocs) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -992,6 +1058,7 @@ int hd_autotune(hd_ctx *ctx, char *report, size_t n) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
     std::string r = "{";
+    drop_graphs(ctx);
     int rc = hd::tune_line_solvers(ctx, ctx->ds, ctx->hs, ctx->nscen, r);
     if (rc) return rc;
     // choices in the order uz, vz, fs_j, fs_i
@@ -1023,6 +1090,7 @@ int hd_set_solver(hd_ctx *ctx, int dir, int hydraulic, int kind, int param) {
                        "that is a multiple of 64 up to 512 (0 = default)");
     ctx->solver_kind[dir][hydraulic ? 1 : 0] = kind;
     ctx->solver_param[dir][hydraulic ? 1 : 0] = param;
+    drop_graphs(ctx);
     return HD_OK;
 }
 
@@ -1051,6 +1119,7 @@ int hd_gtsv_batch(const double *dl, const double *d, const double *du, const dou
 int hd_set_graph(hd_ctx *ctx, int on) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
     if (on && !kGraphEnabled) return set_err(ctx, HD_E_UNSUPPORTED, "whole-step graphs are switched off in this build");
+    if ((on != 0) != ctx->use_graph) drop_graphs(ctx);
     ctx->use_graph = on != 0;
     return HD_OK;
 }

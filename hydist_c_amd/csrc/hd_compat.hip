@@ -37,6 +37,29 @@ void check(int rc, const char *what, hd_ctx *ctx) {
     if (rc != HD_OK) fatal(what, ctx);
 }
 
+// HYDIST_FORM: plain | fast | fp32, or the digits 0-2 (HD_FORM_*); unset
+// means plain.  Anything else ends the program with status 2 before any
+// device work, so a misspelt form never runs silently as another.
+int form_from_env(const char *v) {
+    if (!v || !*v) return HD_FORM_FP64_PLAIN;
+    const std::string s(v);
+    if (s == "plain" || s == "0") return HD_FORM_FP64_PLAIN;
+    if (s == "fast" || s == "1") return HD_FORM_FP64_FAST;
+    if (s == "fp32" || s == "2") return HD_FORM_FP32_REFINE;
+    std::cout << "error: HYDIST_FORM=" << s << ": expected plain, fast or fp32 (or 0, 1, 2)" << std::endl;
+    std::exit(2);
+}
+
+// The drop-in path keeps the reference's literal end test of the
+// open-start branch (dkBienQ_1, UVZSolver_multithread.cpp:438) unless
+// HYDIST_Q11=defined asks for the end side's own flag.
+int quirks_from_env(const char *v) {
+    if (!v || !*v || std::string(v) == "literal") return HD_QUIRK_Q11_LITERAL;
+    if (std::string(v) == "defined") return 0;
+    std::cout << "error: HYDIST_Q11=" << v << ": expected literal or defined" << std::endl;
+    std::exit(2);
+}
+
 hd_ctx *lookup(const void *key) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_by_dev.find(key);
@@ -217,8 +240,8 @@ Argument_Pointers attribute_arrays_memory_alloc(int device, Host_arrays &ap, Arg
     ops.sediment_start = 1e300;
     ops.bed_change_start = 1e300;
     hd_config cfg{};
-    const char *form = std::getenv("HYDIST_FORM");
-    cfg.form = form ? std::atoi(form) : HD_FORM_FP64_PLAIN;
+    cfg.form = form_from_env(std::getenv("HYDIST_FORM"));
+    cfg.quirks = quirks_from_env(std::getenv("HYDIST_Q11"));
     cfg.sediment = 1;
     cfg.bed_change = 0;  // the reference never launches BedLoad (engine.cpp:290-293)
     cfg.device = device;

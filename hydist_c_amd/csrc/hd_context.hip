@@ -133,6 +133,7 @@ int alloc_shared(hd_ctx *ctx, int nscen) {
     A((void **)&B.sys0, nsys * nscen * sizeof(int));
     A((void **)&B.sysn, nsys * nscen * sizeof(int));
     A((void **)&ctx->clock, 64);
+    A((void **)&ctx->wd_flag, 64);
     A((void **)&ctx->ds, sizeof(DevState) * nscen);
     if (rc) return rc;
     ctx->nscen = nscen;
@@ -163,7 +164,7 @@ size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
                                std::max((size_t)(M + 3) * round_up(rowsU, 64), (size_t)(N + 3) * round_up(rowsV, 64)));
     const size_t stride = (sw + 63) / 64 * 64;
     const size_t nsys = std::max(capV * (M + 3), capH * (N + 3));
-    b += 7 * stride * nscen * 8 + stride * nscen + 2 * nsys * nscen * 4 + 64 + sizeof(DevState) * nscen;
+    b += 7 * stride * nscen * 8 + stride * nscen + 2 * nsys * nscen * 4 + 64 + 64 + sizeof(DevState) * nscen;
     return b;
 }
 

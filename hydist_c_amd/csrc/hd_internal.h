@@ -39,6 +39,11 @@ struct hd_ctx {
     size_t bytes = 0;
     bool inited = false;
     int fault_stage = 0;           // test hook: 1 = fail UZ half-step, 2 = fail VZ
+    // NaN/Inf watchdog: a check after every wd_every-th step into wd_flag
+    int wd_every = 0;
+    long wd_steps = 0;
+    bool wd_tripped = false;
+    int *wd_flag = nullptr;
     std::string err;
     // stage profiling: events recorded on `stream` around each launch
     bool prof = false;
@@ -63,6 +68,7 @@ double *field_ptr(const DevState &s, int f);
 Launch2D grid2d(const DevState &s, int nscen);
 int sync_desc(hd_ctx *ctx);
 int check_launch(hd_ctx *ctx, const char *what);
+void drop_graphs(hd_ctx *ctx);
 hipEvent_t prof_begin(hd_ctx *ctx);
 void prof_end(hd_ctx *ctx, const char *what, hipEvent_t e0);
 int prof_resolve(hd_ctx *ctx);

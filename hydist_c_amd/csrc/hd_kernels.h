@@ -32,5 +32,6 @@ __global__ void k_qb(const DevState *S);
 template <bool UPDATE_FS> __global__ void k_bedload(const DevState *S);
 __global__ void k_volume(const DevState *S);
 __global__ void k_watchdog(const DevState *S, int *flag);
+__global__ void k_set_flag(int *flag, int v);
 
 }  // namespace hd

@@ -861,14 +861,22 @@ __global__ void __launch_bounds__(1024) k_volume(const DevState *__restrict__ S)
     }
 }
 
-// NaN/Inf watchdog over the water level: sets flag[0] = 1 when found.
+// NaN/Inf watchdog: sets flag[0] = 1 when z, u or v of a wet compute cell
+// is not finite.
 __global__ void k_watchdog(const DevState *__restrict__ S, int *flag) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     if (i < 2 || j < 2 || i > s.N || j > s.M) return;
     const int pos = i * s.P + j;
-    if (s.khouot[pos] == 0 && !isfinite(s.z[pos])) atomicOr(flag, 1);
+    if (s.khouot[pos] == 0 && !(isfinite(s.z[pos]) && isfinite(s.u[pos]) && isfinite(s.v[pos])))
+        atomicOr(flag, 1);
+}
+
+// Sets a flag word (a kernel rather than a memset, so it can sit in a
+// captured graph).
+__global__ void k_set_flag(int *flag, int v) {
+    if (threadIdx.x == 0) flag[0] = v;
 }
 
 }  // namespace hd

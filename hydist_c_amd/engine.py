@@ -67,6 +67,8 @@ def lib():
         L.hd_restore.argtypes = [vp, C.c_void_p, C.c_size_t, dp]
         L.hd_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_double]
         L.hd_load_checkpoint.argtypes = [vp, C.c_char_p, dp]
+        L.hd_set_watchdog.argtypes = [vp, C.c_int]
+        L.hd_inject_fault.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -145,7 +147,9 @@ class Simulation:
         returning True stops the run."""
         t = C.c_double(self.t)
         if callback is None:
-            check(lib().hd_run(self.h, C.byref(t), Tmax), self.h)
+            rc = lib().hd_run(self.h, C.byref(t), Tmax)
+            self.t = t.value  # the time reached, also when the run stopped early
+            check(rc, self.h)
         else:
             CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_double, C.c_void_p)
             cb = CB(lambda h, tt, u: int(bool(callback(tt))))
@@ -229,6 +233,14 @@ class Simulation:
         v, q = C.c_double(), C.c_double()
         check(lib().hd_mass_balance(self.h, C.byref(v), C.byref(q)), self.h)
         return v.value, q.value
+
+    def set_watchdog(self, every):
+        """NaN/Inf check of z, u, v every `every` steps (0 = off)."""
+        check(lib().hd_set_watchdog(self.h, int(every)), self.h)
+
+    def inject_fault(self, stage):
+        """Arms a failure of the next UZ (1) or VZ (2) half-step; 0 clears."""
+        check(lib().hd_inject_fault(self.h, int(stage)), self.h)
 
 
 def autotune(sim):

@@ -223,6 +223,24 @@ int hd_get_segments(hd_ctx *ctx, int dir, int32_t *counts, size_t ncounts,
  * the device in a fixed order. */
 int hd_mass_balance(hd_ctx *ctx, double *volume, double *inflow);
 
+/* NaN/Inf watchdog (the reference checks only for launch errors after
+ * every kernel, synch_and_check, engine.cpp:13-19).  With every > 0, after
+ * every `every`-th full step (hd_step, hd_run, hd_run_cb; eager or graph)
+ * one kernel checks z, u and v of every wet compute cell of every
+ * scenario.  hd_run and hd_run_cb wait for each check and stop with
+ * HD_E_NUMERIC at the first that finds a non-finite value; after hd_step
+ * the finding is returned by the next hd_sync.  Every later hd_sync,
+ * hd_run and hd_run_cb returns HD_E_NUMERIC too, until the watchdog is
+ * set again (which clears the finding).  every = 0 switches it off (the
+ * default). */
+int hd_set_watchdog(hd_ctx *ctx, int every);
+
+/* Fault injection for failure-handling tests (no counterpart in the
+ * reference): stage 1 makes the next UZ half-step, stage 2 the next VZ
+ * half-step, return HD_E_FAULT before it launches anything; 0 clears it.
+ * The fault stays armed until cleared. */
+int hd_inject_fault(hd_ctx *ctx, int stage);
+
 /* Frees every buffer of the context. */
 int hd_destroy(hd_ctx *ctx);
 

@@ -5,9 +5,13 @@
 //   -f DIR  input directory (trailing '/')   -h/-m/-s  run length
 //   -i N    save interval (s)                -b T      sediment start (s)
 //   -B T    bed-change start (s)
+//   -d FILE after the run, write Z, U, V (full (N+3) x (M+3) grids of
+//           doubles, in that order) to FILE for comparisons at full
+//           precision
 // Outputs land in ./Outputs/{U,V,Z,FS}/ as in loader.cpp:397-434.
 #include <unistd.h>
 
+#include <cstdio>
 #include <cstdlib>
 #include <iostream>
 #include <string>
@@ -16,11 +20,11 @@
 #include "hydist_compat.h"
 
 int main(int argc, char **argv) {
-    std::string dir;
+    std::string dir, dump;
     int hour = 1, minute = 0, sec = 0, save_interval = 60;
     DOUBLE sediment_start = 20 * 3600, bed_change_start = 500 * 3600;
     int c;
-    while ((c = getopt(argc, argv, "f:h:m:s:b:B:i:")) != -1) {
+    while ((c = getopt(argc, argv, "f:h:m:s:b:B:i:d:")) != -1) {
         switch (c) {
             case 'f': dir = optarg; break;
             case 'h': hour = std::atoi(optarg); break;
@@ -29,6 +33,7 @@ int main(int argc, char **argv) {
             case 'b': sediment_start = std::atoi(optarg); break;
             case 'B': bed_change_start = std::atoi(optarg); break;
             case 'i': save_interval = std::atoi(optarg); break;
+            case 'd': dump = optarg; break;
             default: return 2;
         }
     }
@@ -77,5 +82,17 @@ int main(int argc, char **argv) {
     Hydraulic_Calculation(coeffs.dT, coeffs.NANGDAY, d_arg, d_arr, &coeffs, h_arg, ops);
     synch_and_check();
     std::cout << "done t = " << Tmax << std::endl;
+    if (!dump.empty()) {
+        hd_ctx *ctx = hydist_compat_context(&h_arg);
+        const size_t n = (size_t)(M + 3) * (N + 3);
+        std::vector<double> buf(n);
+        FILE *f = std::fopen(dump.c_str(), "wb");
+        if (!ctx || !f) return 3;
+        for (int field : {HD_F_Z, HD_F_U, HD_F_V}) {
+            if (hd_copy_out(ctx, field, buf.data(), n) != HD_OK) return 3;
+            if (std::fwrite(buf.data(), sizeof(double), n, f) != n) return 3;
+        }
+        if (std::fclose(f) != 0) return 3;
+    }
     return 0;
 }

@@ -1,9 +1,17 @@
-"""The reference's host surface (include/hydist_compat.h) on the shipped
-real-river case: a driver written against load_inputs /
-attribute_arrays_memory_alloc / Hydraulic_Calculation / save_result runs
-five simulated minutes, and its text output matches the C-ABI path and the
-oracle on the same inputs (to the 6 significant digits save_result
-writes, loader.cpp:384-388)."""
+"""The reference's host surface (include/hydist_compat.h): a driver
+written against load_inputs / attribute_arrays_memory_alloc /
+Hydraulic_Calculation / save_result (tests/compat/compat_driver.cpp).
+
+- The shipped real-river case runs five simulated minutes; its text output
+  matches the C-ABI path and the oracle on the same inputs (to the 6
+  significant digits save_result writes, loader.cpp:384-388).
+- The drop-in path keeps the reference's literal Q-flag end test
+  (UVZSolver_multithread.cpp:438) by default: on a mesh with a discharge
+  side and rows that open on a water-level side, its full-precision
+  fields equal the oracle run with that quirk, bit for bit, and differ
+  from the oracle without it.
+- HYDIST_FORM=fast / fp32 select the fast forms on the drop-in path;
+  their fields agree with the oracle within the fast forms' bars."""
 import os
 import subprocess
 
@@ -11,8 +19,9 @@ import numpy as np
 import pytest
 
 import oracle_lib as ol
-from hydist_c_amd import engine, luy
-from parity_util import HYDRO, diff_report, make_pair
+from hydist_c_amd import engine, inputs, luy, scenarios as S
+from hydist_c_amd.abi import HD_QUIRK_Q11_LITERAL
+from parity_util import HYDRO, bits_equal, diff_report, make_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -38,7 +47,7 @@ def test_compat_driver_real_river(tmp_path):
     # save_result after the VZ half-step runs before update_uvz
     # (engine.cpp:243-248): z_300 holds the state after the UZ half of t=299
     sc = luy.scenario()
-    g, o = make_pair(sc)
+    g, o = make_pair(sc, quirks=HD_QUIRK_Q11_LITERAL)
     g.step(149)
     o.step(149)
     g.halfstep_uz(299.0)
@@ -51,3 +60,78 @@ def test_compat_driver_real_river(tmp_path):
     assert np.allclose(z_drv, z, rtol=1e-5, atol=1e-12)
     assert np.allclose(u_drv, u, rtol=1e-5, atol=1e-12)
     assert os.path.exists(tmp_path / "Outputs" / "Z" / "z_60.txt")
+
+
+def run_driver(tmp_path, sc, seconds, env_extra=None):
+    """Runs the compat driver on `sc` for `seconds` simulated seconds and
+    returns its full-precision Z, U, V."""
+    d = inputs.write_scenario_inputs(sc, str(tmp_path / "Inputs"))
+    for f in ("U", "V", "Z", "FS"):
+        os.makedirs(tmp_path / "Outputs" / f, exist_ok=True)
+    dump = str(tmp_path / "fields.bin")
+    env = dict(os.environ)
+    env.update(env_extra or {})
+    r = subprocess.run([DRIVER, "-f", d, "-h", "0", "-m", "0", "-s", str(seconds), "-i", "100000", "-d", dump],
+                       cwd=str(tmp_path), capture_output=True, text=True, timeout=1500 if EMU else 240, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    a = np.fromfile(dump, dtype=np.float64).reshape(3, *sc.shape)
+    return {"Z": a[0], "U": a[1], "V": a[2]}
+
+
+def oracle_fields(sc, steps, quirks):
+    o = ol.Oracle(sc, cfg=ol.make_config(sediment=True, bed_change=False, quirks=quirks))
+    o.init()
+    # the drop-in path takes Kx1, Ky1, hi and hmax from the device's pow()
+    g = engine.Simulation(sc, sediment=True, quirks=quirks)
+    g.init()
+    for f in ("KX1", "KY1"):
+        o.set(f, g.get(f))
+    o.set_boundary("HI", g.get_boundary("HI"))
+    o.set_hmax(g.get_hmax())
+    g.close()
+    o.step(steps)
+    return {f: o.get(f) for f in ("Z", "U", "V")}
+
+
+def q11_box():
+    # a discharge side (up, j = M) and water-level sides elsewhere: rows
+    # running from j = 2 to j = M are where the literal test acts
+    return S.open_box(40, 28, types=(2, 0, 0, 0))
+
+
+def test_compat_driver_keeps_q11_literal(tmp_path):
+    sc = q11_box()
+    print("[case] compat driver, Q11 literal by default, open box 40x28, 20 s", flush=True)
+    got = run_driver(tmp_path, sc, 20)
+    lit = oracle_fields(sc, 10, HD_QUIRK_Q11_LITERAL)
+    dfn = oracle_fields(sc, 10, 0)
+    for f in ("Z", "U", "V"):
+        assert bits_equal(got[f], lit[f]), f
+    assert not all(bits_equal(lit[f], dfn[f]) for f in ("Z", "U", "V")), "the case does not exercise Q11"
+
+
+def test_compat_driver_q11_defined_opt_out(tmp_path):
+    sc = q11_box()
+    print("[case] compat driver, HYDIST_Q11=defined, open box 40x28, 20 s", flush=True)
+    got = run_driver(tmp_path, sc, 20, {"HYDIST_Q11": "defined"})
+    dfn = oracle_fields(sc, 10, 0)
+    for f in ("Z", "U", "V"):
+        assert bits_equal(got[f], dfn[f]), f
+
+
+# bars of the fast forms on the drop-in path after 30 steps of config 1
+COMPAT_TOL = {"fast": 1e-12, "fp32": 1e-9}
+
+
+@pytest.mark.parametrize("form", ["fast", "fp32"])
+def test_compat_driver_fast_forms(tmp_path, form):
+    sc = S.config1()
+    print(f"[case] compat driver, HYDIST_FORM={form}, config 1, 60 s", flush=True)
+    got = run_driver(tmp_path, sc, 60, {"HYDIST_FORM": form})
+    ref = oracle_fields(sc, 30, HD_QUIRK_Q11_LITERAL)
+    for f in ("Z", "U", "V"):
+        d = float(np.max(np.abs(got[f] - ref[f])))
+        print(f"[compat {form}] max|d{f}| = {d:.3e}", flush=True)
+        assert d <= COMPAT_TOL[form], (f, d)
+    # not the plain form: the fast forms round differently somewhere
+    assert not all(bits_equal(got[f], ref[f]) for f in ("Z", "U", "V"))
