@@ -33,46 +33,87 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def sweep_unknowns(sim, direction):
-    """Interleaved (z, velocity) unknowns of one sweep: 2L - 1 per wet
-    segment of L cells (exact for wall-ended segments, +-1 otherwise)."""
-    counts, first, last = sim.segments(direction)
-    L = (last - first + 1).astype("int64")
-    return int((2 * L - 1).sum())
+def line_units(sim, sc):
+    """Counts over the current wet-segment tables that the line kernels'
+    algorithmic bytes are priced on.  For direction d (0: row lines, the
+    VZ sweep and FS along i; 1: column lines, UZ and FS along j), over the
+    segments of L cells:
+      wet_d   sum L        positions whose state a line kernel reads
+      il_d    sum 2L - 1   unknowns of the interleaved (z, velocity) systems
+      fs_d    sum L - 2    FS unknowns (segments of 3 or more cells)
+      fsc_d   sum L        cells of those FS segments (extraction writes)
+    and pos_d, the line positions of a sweep over the whole mesh
+    (identity rows included), for the all-positions figures."""
+    u = {}
+    for d in (0, 1):
+        counts, first, last = sim.segments(d)
+        L = (last - first + 1).astype("int64")
+        long_ = L[L >= 3]
+        u[f"wet_{d}"] = int(L.sum())
+        u[f"il_{d}"] = int((2 * L - 1).sum())
+        u[f"fs_{d}"] = int((long_ - 2).sum())
+        u[f"fsc_{d}"] = int(long_.sum())
+        u[f"pos_{d}"] = (sc.M - 1) * (sc.N - 1)
+    return u
 
 
-def stage_bytes(name, sim, sc):
-    """Algorithmic bytes of one launch of a stage (FP64)."""
+# Algorithmic bytes per unit of the line kernels: (unit, bytes).  FP64
+# values: the hydraulic assembly reads 12 state/static fields and writes 2
+# rows x 4 coefficients; a solve reads 4 coefficients and writes 1 value
+# per unknown; extraction reads 2 solution values and writes t_z and the
+# velocity; FS assembly reads 12 fields and writes 4 coefficients; FS
+# extraction reads the solution and FS and writes tFS.  The one-kernel
+# row-line sweeps read the assembly's fields and write the extraction's
+# outputs: VZ 12 in + 2 out per wet position; FS along i 12 in per FS
+# unknown + tFS out per cell of the segment.
+LINE_PRICES = {
+    "hyd_assemble": ("wet", 160.0),
+    "hyd_solve": ("il", 40.0),
+    "hyd_extract": ("wet", 32.0),
+    "vz_fused": ("wet", 112.0),
+    "fs_assemble": ("fs", 128.0),
+    "fs_solve": ("fs", 40.0),
+    "fs_extract": ("fs", 24.0),
+}
+# the same per position of every line (identity rows included): the
+# all-positions figures of the row and column sweeps
+ALL_POS_PRICES = {"hyd_assemble": 160.0, "hyd_solve": 80.0, "hyd_extract": 32.0, "vz_fused": 112.0,
+                  "fs_assemble": 128.0, "fs_solve": 40.0, "fs_extract": 24.0, "fs_i_fused": 104.0}
+
+
+def line_kind(name):
+    """(kind, direction) of a line-kernel stage name, or None."""
+    if not name.startswith(("k_cr", "k_part", "k_thomas", "k_hyd_assemble", "k_hyd_extract", "k_fs_assemble",
+                            "k_fs_extract")):
+        return None
+    d = 1 if any(t in name for t in ("uz", "fs_j", "<j>")) else 0
+    hyd = "fs" not in name
+    if name.startswith("k_part<") and name.endswith(",vz>"):
+        return "vz_fused", 0
+    if name.startswith("k_part<") and name.endswith(",fs_i>"):
+        return "fs_i_fused", 0
+    if name.startswith(("k_cr", "k_part", "k_thomas")):
+        return ("hyd_solve" if hyd else "fs_solve"), d
+    for k in ("hyd_assemble", "hyd_extract", "fs_assemble", "fs_extract"):
+        if name.startswith("k_" + k):
+            return k, d
+    return None
+
+
+def stage_bytes(name, units, sc, all_positions=False):
+    """Algorithmic bytes of one launch of a stage (FP64 state).  Line
+    kernels are priced on the wet segments (line_units); with
+    all_positions, on every position of every line instead."""
     cells = (sc.M + 3) * (sc.N + 3)
-    if name.startswith("k_sweep_plain"):
-        d = 1 if "uz" in name else 0
-        return 40.0 * sweep_unknowns(sim, d)  # (a, b, c, d in + x out) * 8 B per unknown
-    # line sweeps over (L-1) positions of every line
-    lines_uz, lines_vz = sc.M - 1, sc.N - 1
-    pos_uz, pos_vz = sc.N - 1, sc.M - 1
-    if name.startswith(("k_cr", "k_part", "k_thomas", "k_hyd_assemble", "k_hyd_extract", "k_fs_assemble",
-                        "k_fs_extract")):
-        uz = any(t in name for t in ("uz", "fs_j", "<j>"))
-        n = (lines_uz * pos_uz) if uz else (lines_vz * pos_vz)
-        hyd = "fs" not in name
-        if name.startswith("k_part<") and name.endswith(",vz>"):
-            # the VZ sweep in one kernel (rows assembled, solved and
-            # extracted on chip): the assembly's 12 state/static fields in,
-            # t_z and t_v out
-            return (12 * 8 + 2 * 8) * float(n)
-        if name.startswith("k_part<") and name.endswith(",fs_i>"):
-            # the FS sweep along i in one kernel: 12 fields in, tFS out
-            return (12 * 8 + 8) * float(n)
-        if name.startswith(("k_cr", "k_part", "k_thomas")):
-            # tridiagonal solve: 4 coefficients in + solution out per unknown
-            return (80.0 if hyd else 40.0) * n
-        if name.startswith("k_hyd_assemble"):
-            return (12 * 8 + 2 * 4 * 8) * float(n)
-        if name.startswith("k_fs_assemble"):
-            return (12 * 8 + 4 * 8) * float(n)
-        if name.startswith("k_hyd_extract"):
-            return 32.0 * n
-        return 24.0 * n
+    lk = line_kind(name)
+    if lk is not None:
+        kind, d = lk
+        if all_positions:
+            return ALL_POS_PRICES[kind] * units[f"pos_{d}"]
+        if kind == "fs_i_fused":
+            return 96.0 * units["fs_0"] + 8.0 * units["fsc_0"]
+        unit, price = LINE_PRICES[kind]
+        return price * units[f"{unit}_{d}"]
     per_cell = {
         "k_smooth": 8 + 4 + 8,        # t_vel + khouot in, tmp out
         "k_solve_v": 10 * 8 + 4 + 16,  # stencil fields in, t_v + t_u out
@@ -86,7 +127,6 @@ def stage_bytes(name, sim, sc):
         "k_qb": 7 * 8 + 4 + 16,
         "k_bedload": 12 * 8 + 4 + 16,
         "k_segment": 8,               # khouot in, once per direction (segment tables are O(lines))
-        "k_dry": 8 + 4,
     }
     if name.startswith("k_rewet"):
         # candidate mask words in (the marked rows' reads are data dependent)
@@ -94,20 +134,6 @@ def stage_bytes(name, sim, sc):
     for k, v in per_cell.items():
         if name.startswith(k):
             return float(v) * cells
-    return None
-
-
-def measured_traffic(symbol):
-    """HBM bytes per launch of a kernel from the committed rocprofv3
-    counter summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", "c2_2048x1024_fast_hbm_traffic.md")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch "):
-            name, value = line[len("traffic_bytes_per_launch "):].rsplit(" ", 1)
-            if name.strip() == symbol:
-                return float(value)
     return None
 
 
@@ -181,17 +207,21 @@ def time_form(sc, form, steps, warmup, profile_steps):
     engine.set_profiling(sim, False)
     z = sim.get("Z")
     finite = bool((z == z).all())
-    return sim, ms_per_step, stats, finite, tune
+    volume, inflow = sim.mass_balance()
+    mass = {"t": sim.t, "volume_m3": volume, "inflow_m3": inflow}
+    return sim, ms_per_step, stats, finite, tune, mass
 
 
 def cpu_baseline(sc, seconds, threads=None):
-    """The oracle (OpenMP over lines) on this host, on the same mesh with
-    every stage active (sediment and bed change from t = 0), for a bounded
-    number of steps."""
+    """The oracle (OpenMP over lines) on this host, built -O3 -march=native
+    for this CPU (a timing copy; the checker is the -O2 -ffp-contract=off
+    build), on the same mesh with every stage active (sediment and bed
+    change from t = 0), for a bounded number of steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     from hydist_c_amd.abi import make_config
     import copy
+    model = ol.use_native_build()
     s2 = copy.copy(sc)
     s2.sediment_start = 0.0
     s2.bed_change_start = 0.0
@@ -204,11 +234,12 @@ def cpu_baseline(sc, seconds, threads=None):
     t0 = time.perf_counter()
     o.step(1)
     one = time.perf_counter() - t0
-    n = max(1, min(200, int(seconds / max(one, 1e-6))))
+    n = max(1, min(400, int(seconds / max(one, 1e-6))))
     t0 = time.perf_counter()
     o.step(n)
     dt = time.perf_counter() - t0
     return {"value": sc.M * sc.N * n / dt, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(), "build": "g++ -O3 -march=native -fopenmp",
             "sample": f"{n} full steps (hydraulics + sediment + bed change) of the {sc.M}x{sc.N} mesh, "
                       f"{dt:.1f} s on {threads} threads"}
 
@@ -388,57 +419,69 @@ def main():
     for name in names:
         log(f"[bench] form {name}: warm-up to t={sc.sediment_start:.0f}s + {args.warmup} steps, "
             f"then {args.steps} timed steps")
-        sim, ms, stats, finite, tune = time_form(sc, forms[name], args.steps, args.warmup, args.profile_steps)
+        sim, ms, stats, finite, tune, mass = time_form(sc, forms[name], args.steps, args.warmup,
+                                                       args.profile_steps)
         cs = sc.M * sc.N / (ms * 1e-3)
         results[name] = {"cell_steps_per_s": cs, "ms_per_step": ms, "finite": finite, "stats": stats, "sim": sim,
-                         "tune": tune}
+                         "tune": tune, "mass": mass}
         log(f"[bench] form {name}: {ms:.3f} ms/step, {cs:.3e} cell-steps/s")
         if best is None or cs > results[best]["cell_steps_per_s"]:
             best = name
+        if name != names[-1]:
+            sim.close()
     r = results[best]
     sim = r["sim"]
     stats = r["stats"]
+    units = line_units(sim, sc)
+    chosen = r["tune"]["chosen"] if r["tune"] else None
+    layout = 0 if best == "plain" else 1
     # dominant kernel symbol by total time (a symbol can serve several
     # stages, e.g. one cyclic-reduction kernel for both hydraulic sweeps)
     groups = {}
     for name, (launches, total_ms) in stats.items():
         if launches == 0 or total_ms <= 0:
             continue
-        chosen = r["tune"]["chosen"] if r["tune"] else None
-        g = groups.setdefault(kernel_symbol(name, sc, chosen, 0 if best == "plain" else 1), [0, 0.0, 0.0, True])
-        b = stage_bytes(name, sim, sc)
+        g = groups.setdefault(kernel_symbol(name, sc, chosen, layout), [0, 0.0, 0.0, 0.0, True, []])
+        b = stage_bytes(name, units, sc)
+        ball = stage_bytes(name, units, sc, all_positions=True)
         g[0] += launches
         g[1] += total_ms
+        g[5].append(name)
         if b is None:
-            g[3] = False
+            g[4] = False
         else:
             g[2] += b * launches
+            g[3] += ball * launches
     roofline = None
     if groups:
-        dom_name, (launches, total_ms, total_bytes, known) = max(groups.items(), key=lambda kv: kv[1][1])
+        dom_name, (launches, total_ms, total_bytes, total_all, known, stage_names) = max(
+            groups.items(), key=lambda kv: kv[1][1])
         avg_ms = total_ms / launches
-    if groups and known:
-        nbytes = total_bytes / launches
-        achieved = nbytes / (avg_ms * 1e-3) / 1e9
-        traffic = measured_traffic(dom_name)
-        roofline = {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                    "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms, "launches": launches}
-        if traffic is not None:
-            roofline["traffic_source"] = "profiles/c2_2048x1024_fast_hbm_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
-    # the tridiagonal solve kernel of the best form, at its own roofline.  The
-    # partition solver of the VZ sweep also assembles its rows and extracts
-    # t_z and t_v (the whole sweep is one kernel), so the pure solve reported
-    # is the slowest other hydraulic one; the fused kernel is reported beside it.
+        if known:
+            nbytes = total_bytes / launches
+            achieved = nbytes / (avg_ms * 1e-3) / 1e9
+            all_b = total_all / launches
+            roofline = {"bound": "hbm", "kernel": dom_name, "stages": stage_names, "achieved": achieved,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "bytes_per_launch": nbytes, "avg_launch_ms": avg_ms, "launches": launches,
+                        "priced_on": "wet segments (line kernels) / all cells (stencil kernels)",
+                        "all_positions": {"bytes_per_launch": all_b,
+                                          "achieved": all_b / (avg_ms * 1e-3) / 1e9,
+                                          "frac": all_b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    # the tridiagonal solve kernel of the best form, at its own roofline:
+    # the slowest pure hydraulic solve, priced at 40 B per interleaved
+    # unknown of the wet segments; the one-kernel VZ sweep beside it
     tri = None
     solver = [(k, v) for k, v in stats.items()
               if k.startswith(("k_cr", "k_part", "k_thomas")) and "fs" not in k and v[0] > 0 and v[1] > 0]
 
     def solver_roof(k, launches, total_ms):
-        b = stage_bytes(k, sim, sc)
-        a = b / (total_ms / launches * 1e-3) / 1e9
-        return {"kernel": k, "bytes_per_launch": b, "avg_launch_ms": total_ms / launches, "achieved": a,
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS}
+        b = stage_bytes(k, units, sc)
+        ball = stage_bytes(k, units, sc, all_positions=True)
+        t = total_ms / launches * 1e-3
+        return {"kernel": k, "bytes_per_launch": b, "avg_launch_ms": t * 1e3, "achieved": b / t / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b / t / 1e9 / HBM_PEAK_GBS,
+                "all_positions_frac": ball / t / 1e9 / HBM_PEAK_GBS}
 
     fused = [(k, v) for k, v in solver if k.startswith("k_part<") and k.endswith(",vz>")]
     pure = [(k, v) for k, v in solver if (k, v) not in fused]
@@ -449,7 +492,7 @@ def main():
         k, (launches, total_ms) = fused[0]
         f = solver_roof(k, launches, total_ms)
         f["note"] = ("the VZ sweep in one kernel: rows assembled from the state, solved and t_z, t_v extracted "
-                     "on chip; bytes = the 12 state/static fields in + t_z, t_v out")
+                     "on chip; bytes = the 12 state/static fields in + t_z, t_v out per wet position")
         if tri is None:
             tri = f
         else:
@@ -457,13 +500,20 @@ def main():
     step_bytes = STEP_BYTES["bed_change"] * sc.M * sc.N
     step_roof = {"bytes_per_step": step_bytes, "achieved_GBs": step_bytes / (r["ms_per_step"] * 1e-3) / 1e9}
     step_roof["frac"] = step_roof["achieved_GBs"] / HBM_PEAK_GBS
+    # domain volume of each form at the same time, against the plain form
+    # (bit for bit the oracle's hydraulics)
+    mass = {k: dict(v["mass"]) for k, v in results.items()}
+    if "plain" in mass:
+        v0 = mass["plain"]["volume_m3"]
+        for k, v in mass.items():
+            v["rel_diff_vs_plain"] = (v["volume_m3"] - v0) / v0 if v0 else None
     ensemble = None
     if args.ensemble > 0:
         log(f"[bench] ensemble: {args.ensemble} x 512x256 in one batched context, {args.ensemble_steps} steps per form")
         ensemble = ensemble_measure(args.ensemble, args.ensemble_steps)
         log("[bench] ensemble: " + ", ".join(f"{f} {ensemble[f]['ms_per_step']:.2f} ms/step "
                                              f"{ensemble[f]['cell_steps_per_s']:.3e} cell-steps/s"
-                                             for f in ("plain", "fast", "fp32")))
+                                             for f in ("plain", "fast", "fp32") if f in ensemble))
     serving = None
     if args.serving_seconds > 0:
         log(f"[bench] serving loop: {args.serving_rate:.0f} req/s for {args.serving_seconds:.0f} s")
@@ -480,7 +530,7 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle baseline")
         cpu = cpu_baseline(sc, args.cpu_seconds)
-        log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s ({cpu['sample']})")
+        log(f"[bench] cpu: {cpu['value']:.3e} cell-steps/s ({cpu['sample']}, {cpu['cpu_model']})")
         cpu1 = cpu_baseline(sc, args.cpu_seconds / 2, threads=1)
         log(f"[bench] cpu, 1 thread: {cpu1['value']:.3e} cell-steps/s ({cpu1['sample']})")
         cpu["single_thread"] = {"value": cpu1["value"], "sample": cpu1["sample"]}
@@ -505,8 +555,10 @@ def main():
                                              for a, b in v["stats"].items()), key=lambda d: -d["avg_ms"])[:6]}
                   for k, v in results.items()},
         "roofline": roofline,
+        "line_units": units,
         "step_roofline": step_roof,
         "tridiag_roofline": tri,
+        "mass_balance": mass,
         "cpu_baseline": cpu,
         "serving": serving,
         "serving_full": full,

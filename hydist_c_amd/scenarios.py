@@ -367,10 +367,19 @@ def braided(M, N, seed=5, n_channels=12, width=(10, 120), depth=(1.0, 10.0), Q=-
     )
 
 
-def config3(seed=5):
-    """4096 x 4096 braided river (12 random-walk channels), dt = 1 s."""
+def config3(seed=5, sediment=False):
+    """4096 x 4096 braided river (12 random-walk channels), dt = 1 s.  With
+    sediment: suspended sediment from t = 0 with config 2's sediment model
+    (d50 = 0.2 mm, Toe = 0.1, Tod = 0.06 N/m^2) and an up-side FS
+    concentration U[0.05, 0.2] kg/m^3."""
     s = braided(4096, 4096, seed=seed, name="config3_4096x4096")
     s.coeff_overrides = dict(s.coeff_overrides, dT=1.0)
+    if sediment:
+        rng = np.random.default_rng(seed + 500)
+        zeros = _series(0.0, s.total_time)
+        s.CC = [rng.uniform(0.05, 0.2, size=s.total_time), zeros, zeros, zeros]
+        s.coeff_overrides.update({"dm": 2e-4, "Toe": 0.1, "Tod": 0.06})
+        s.sediment_start = 0.0
     return s
 
 
@@ -455,12 +464,25 @@ def tidal_flats(M, N, seed=6, shelf=(0.03, 0.16), channel_depth=3.0, drop=2.0, r
     )
 
 
-def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0.5, p_fp32=0.5, fp64_form=0):
+# A request's tolerance on the water level (m) is log-uniform in
+# [TOL_RANGE]; FP32 + refine serves requests whose tolerance is at least
+# FP32_MIN_TOL (its error after one simulated hour on the 2048 x 1024
+# mesh is a few 1e-12 m), FP64 the stricter half.
+TOL_RANGE = (1e-12, 1e-4)
+FP32_MIN_TOL = 1e-8
+
+
+def form_for_tolerance(tol, fp64_form=0):
+    return 2 if tol >= FP32_MIN_TOL else fp64_form
+
+
+def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0.5, fp64_form=0):
     """What-if request k of the serving workload: a meander + islands mesh
     with a log-uniform side in `side` (the other side 0.5-1.0 of it), a
     log-uniform simulated duration in `duration` (s), sediment on with
-    probability p_sediment, and FP32 + refine (the looser tolerance) with
-    probability p_fp32, else FP64 in form `fp64_form` (0 plain, 1 fast).
+    probability p_sediment, and a log-uniform tolerance in TOL_RANGE that
+    picks FP32 + refine (tolerance >= FP32_MIN_TOL, half the requests) or
+    FP64 in form `fp64_form` (0 plain, 1 fast).
     Returns (scenario, Tmax, form, sediment)."""
     rng = np.random.default_rng(50_000 + k)
     M = int(round(np.exp(rng.uniform(np.log(side[0]), np.log(side[1])))))
@@ -468,10 +490,12 @@ def serving_request(k, side=(128, 2048), duration=(600.0, 21600.0), p_sediment=0
     T = float(np.exp(rng.uniform(np.log(duration[0]), np.log(duration[1]))))
     T = 2.0 * np.ceil(T / 2.0)
     sed = bool(rng.random() < p_sediment)
-    form = 2 if rng.random() < p_fp32 else fp64_form
+    tol = float(np.exp(rng.uniform(np.log(TOL_RANGE[0]), np.log(TOL_RANGE[1]))))
+    form = form_for_tolerance(tol, fp64_form)
     n_isl = max(1, int(30 * M * N / (2048 * 1024)))
     s = meander_islands(M, N, seed=k, n_islands=n_isl, Q=-4000.0 * N / 1024, hours=int(np.ceil(T / 3600.0)),
                         sediment_start=0.0, name=f"request{k}_{M}x{N}")
+    s.meta["tolerance"] = tol
     return s, T, form, sed
 
 

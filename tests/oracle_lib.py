@@ -15,11 +15,37 @@ from hydist_c_amd.abi import F, MeshHolder, make_config, make_options
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libhydist_oracle.so")
 
+NATIVE_PATH = os.path.join(ROOT, "oracle", "_build", "native", "libhydist_oracle.so")
+
 _lib = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+
+
+def cpu_model():
+    with open("/proc/cpuinfo") as f:
+        for line in f:
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    return "unknown"
+
+
+def use_native_build():
+    """Switches this process to the -O3 -march=native copy of the oracle
+    (timing only: it contracts a*b+c, so its bits differ from the
+    checker's), building it first when it is missing or was built on
+    another CPU.  Returns the CPU model it was built for."""
+    global LIB_PATH, _lib
+    host = os.path.join(os.path.dirname(NATIVE_PATH), "host.txt")
+    model = cpu_model()
+    built_for = open(host).readline().strip() if os.path.exists(host) else None
+    if not os.path.exists(NATIVE_PATH) or built_for != model:
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"], check=True)
+    LIB_PATH = NATIVE_PATH
+    _lib = None
+    return model
 
 
 def lib():

@@ -36,17 +36,44 @@ def test_assembly_and_extraction_symbols_by_layout():
     assert bench.kernel_symbol("k_post_a", SC, None, 1) == "k_post_a"
 
 
-def test_stage_bytes():
-    # hydraulic solve: 80 B per line position, (M-1)(N-1) positions
+def test_stage_bytes_all_positions():
+    # every position of every line, identity rows included: hydraulic solve
+    # 80 B per position, (M-1)(N-1) positions
     pos = (SC.M - 1) * (SC.N - 1)
-    assert bench.stage_bytes("k_part<f64,uz>", None, SC) == 80.0 * pos
-    assert bench.stage_bytes("k_cr_reg<f32r,vz>", None, SC) == 80.0 * pos
-    # the partition solver's VZ sweep assembles and extracts too: 12 fields in, t_z and t_v out
-    assert bench.stage_bytes("k_part<f64,vz>", None, SC) == 112.0 * pos
-    assert bench.stage_bytes("k_part<f64,fs_j>", None, SC) == 40.0 * pos
-    # the FS sweep along i in one kernel: 12 fields in, tFS out
-    assert bench.stage_bytes("k_part<f64,fs_i>", None, SC) == 104.0 * pos
-    assert bench.stage_bytes("k_hyd_assemble<uz>", None, SC) == 160.0 * pos
+    u = {"pos_0": pos, "pos_1": pos}
+    assert bench.stage_bytes("k_part<f64,uz>", u, SC, all_positions=True) == 80.0 * pos
+    assert bench.stage_bytes("k_cr_reg<f32r,vz>", u, SC, all_positions=True) == 80.0 * pos
+    assert bench.stage_bytes("k_part<f64,vz>", u, SC, all_positions=True) == 112.0 * pos
+    assert bench.stage_bytes("k_part<f64,fs_j>", u, SC, all_positions=True) == 40.0 * pos
+    assert bench.stage_bytes("k_part<f64,fs_i>", u, SC, all_positions=True) == 104.0 * pos
+    assert bench.stage_bytes("k_hyd_assemble<uz>", u, SC, all_positions=True) == 160.0 * pos
     cells = (SC.M + 3) * (SC.N + 3)
-    assert bench.stage_bytes("k_segment", None, SC) == 8.0 * cells
-    assert bench.stage_bytes("no_such_kernel", None, SC) is None
+    assert bench.stage_bytes("k_segment", u, SC) == 8.0 * cells
+    assert bench.stage_bytes("no_such_kernel", u, SC) is None
+
+
+class FakeSim:
+    """Segment tables of two row segments (L = 5, 2) and one column
+    segment (L = 4)."""
+    import numpy as np
+
+    def segments(self, d):
+        np = self.np
+        if d == 0:
+            return np.array([0, 2]), np.array([2, 10]), np.array([6, 11])
+        return np.array([1]), np.array([3]), np.array([6])
+
+
+def test_line_units_and_wet_pricing():
+    u = bench.line_units(FakeSim(), SC)
+    assert u["wet_0"] == 7 and u["il_0"] == 9 + 3 and u["fs_0"] == 3 and u["fsc_0"] == 5
+    assert u["wet_1"] == 4 and u["il_1"] == 7 and u["fs_1"] == 2 and u["fsc_1"] == 4
+    # solves: 40 B per interleaved unknown of the wet segments
+    assert bench.stage_bytes("k_part<f64,uz>", u, SC) == 40.0 * 7
+    assert bench.stage_bytes("k_thomas<vz>", u, SC) == 40.0 * 12
+    assert bench.stage_bytes("k_part<f64,fs_j>", u, SC) == 40.0 * 2
+    # one-kernel sweeps: the state of the wet positions in, results out
+    assert bench.stage_bytes("k_part<f64,vz>", u, SC) == 112.0 * 7
+    assert bench.stage_bytes("k_part<f64,fs_i>", u, SC) == 96.0 * 3 + 8.0 * 5
+    assert bench.stage_bytes("k_hyd_assemble<uz>", u, SC) == 160.0 * 4
+    assert bench.stage_bytes("k_hyd_extract<uz>", u, SC) == 32.0 * 4

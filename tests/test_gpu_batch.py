@@ -78,3 +78,29 @@ def test_batch_rejects_mixed_shapes():
     b = S.ensemble_member(1, M=64, N=40)
     with pytest.raises(engine.HydistError):
         engine.Batch([a, b])
+
+
+def test_solver_change_recaptures_graph():
+    """A solver change after graph steps takes effect in the next graph
+    step: the graph run equals an eager run with the same solver sequence,
+    and differs (at rounding level) from a run that kept the first
+    solver."""
+    _case("graph recapture after set_solver")
+    sc = S.open_box(64, 40, seed=11, n_islands=3)
+    runs = []
+    for graph, change in ((True, True), (False, True), (False, False)):
+        g = engine.Simulation(sc, form=abi.HD_FORM_FP64_FAST)
+        if graph:
+            engine.check(engine.lib().hd_set_graph(g.h, 1), g.h)
+        g.init()
+        g.step(3)
+        if change:
+            for d in (0, 1):
+                engine.set_solver(g, d, 1, engine.SOLVER_CR, 256)
+        g.step(4)
+        runs.append({f: g.get(f) for f in ("Z", "U", "V")})
+        g.close()
+    graph_run, eager_run, kept = runs
+    for f in graph_run:
+        assert bits_equal(graph_run[f], eager_run[f]), f
+    assert not all(bits_equal(eager_run[f], kept[f]) for f in kept), "the solvers round alike here"
