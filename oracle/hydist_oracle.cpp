@@ -1836,13 +1836,17 @@ int ho_get_segments(ho_ctx *ctx, int dir, int32_t *counts, size_t ncounts, int32
 int ho_mass_balance(ho_ctx *ctx, double *volume, double *inflow) {
     if (!ctx) return HD_E_INVALID;
     Oracle &o = ctx->o;
-    // row-major order over compute cells, one accumulator
+    // per-row sums in column order, then the rows in order (the order the
+    // library's device reduction uses, so equal states give equal volumes)
     double vol = 0;
-    for (int i = 2; i <= o.N; i++)
+    for (int i = 2; i <= o.N; i++) {
+        double row = 0;
         for (int j = 2; j <= o.M; j++) {
             const int p = i * o.W + j;
-            if (o.khouot[p] == 0) vol += (o.htaiz[p] + o.z[p]) * o.c.dX * o.c.dY;
+            if (o.khouot[p] == 0) row += (o.htaiz[p] + o.z[p]) * o.c.dX * o.c.dY;
         }
+        vol += row;
+    }
     if (volume) *volume = vol;
     if (inflow) *inflow = o.inflow;
     return HD_OK;
