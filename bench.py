@@ -426,8 +426,10 @@ def main():
                          "tune": tune, "mass": mass}
         log(f"[bench] form {name}: {ms:.3f} ms/step, {cs:.3e} cell-steps/s")
         if best is None or cs > results[best]["cell_steps_per_s"]:
+            if best is not None:
+                results[best]["sim"].close()
             best = name
-        if name != names[-1]:
+        else:
             sim.close()
     r = results[best]
     sim = r["sim"]

@@ -120,7 +120,8 @@ def test_compat_driver_q11_defined_opt_out(tmp_path):
 
 
 # bars of the fast forms on the drop-in path after 30 steps of config 1
-COMPAT_TOL = {"fast": 1e-12, "fp32": 1e-9}
+# (observed: fast 2.5e-16, FP32 + refine 2.7e-14)
+COMPAT_TOL = {"fast": 1e-13, "fp32": 3e-12}
 
 
 @pytest.mark.parametrize("form", ["fast", "fp32"])

@@ -38,7 +38,11 @@ pytestmark = pytest.mark.gpu
 
 STEPS = 20
 FAST_STEPS = {"stripes": 1}  # see the module docstring
-TOL = {HD_FORM_FP64_FAST: 1e-9, HD_FORM_FP32_REFINE: 1e-6}
+# hydraulics (m, m/s) and FS / bed change relative to the field's largest
+# value: about 100 times the largest error observed on these cases (fast
+# FP64 ~1e-15, FP32 + refine a few 1e-12 on the hydraulics)
+TOL = {HD_FORM_FP64_FAST: 1e-13, HD_FORM_FP32_REFINE: 5e-10}
+REL_TOL = {HD_FORM_FP64_FAST: 1e-12, HD_FORM_FP32_REFINE: 1e-9}
 NAMES = {HD_FORM_FP64_FAST: "fp64_fast", HD_FORM_FP32_REFINE: "fp32_refine"}
 
 
@@ -130,11 +134,13 @@ def test_edge_mesh_fast_forms(form, mesh):
     assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
     for f in ("Z", "U", "V"):
         d = np.max(np.abs(g.get(f) - o.get(f)))
+        print(f"[{f}] max abs error {d:.3e}", flush=True)
         assert d <= TOL[form], (f, d)
     for f in ("FS", "DH"):
         a, b = g.get(f), o.get(f)
         scale = np.abs(b).max() + 1e-300
-        assert np.max(np.abs(a - b)) <= 1e3 * TOL[form] * scale, (f, np.max(np.abs(a - b)), scale)
+        print(f"[{f}] relative error {np.max(np.abs(a - b)) / scale:.3e}", flush=True)
+        assert np.max(np.abs(a - b)) <= REL_TOL[form] * scale, (f, np.max(np.abs(a - b)), scale)
 
 
 def test_segments_of_a_wide_speckled_mesh():

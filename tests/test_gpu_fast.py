@@ -6,9 +6,11 @@ HD_FORM_FP32_REFINE solves it in FP32 and adds one FP64
 residual-refinement solve.  Both differ from the reference's interleaved
 Thomas only by round-off, so they are compared with tolerances:
 
-  FP64_FAST:   |dZ|, |dU|, |dV| <= 1e-9   after the stated steps
-  FP32_REFINE: |dZ| <= 1e-6 m, |dU|, |dV| <= 1e-6 m/s
+  FP64_FAST:   |dZ|, |dU|, |dV| <= 1e-13; FS, dH within 1e-12 relative
+  FP32_REFINE: |dZ|, |dU|, |dV| <= 5e-10; FS, dH within 1e-9 relative
   wet/dry mask and segment tables: identical
+(the tidal-flats case keeps its own bar, explained there; the one-hour
+run of config 1 is held to the accuracy bar of the mixed form).
 """
 import numpy as np
 import pytest
@@ -19,7 +21,11 @@ from parity_util import make_pair
 
 pytestmark = pytest.mark.gpu
 
-TOL = {HD_FORM_FP64_FAST: 1e-9, HD_FORM_FP32_REFINE: 1e-6}
+# hydraulics (m, m/s) and FS / bed change relative to the field's largest
+# value: about 100 times the largest error observed on these cases (fast
+# FP64 ~1e-15, FP32 + refine a few 1e-12 on the hydraulics)
+TOL = {HD_FORM_FP64_FAST: 1e-13, HD_FORM_FP32_REFINE: 5e-10}
+REL_TOL = {HD_FORM_FP64_FAST: 1e-12, HD_FORM_FP32_REFINE: 1e-9}
 NAMES = {HD_FORM_FP64_FAST: "fp64_fast", HD_FORM_FP32_REFINE: "fp32_refine"}
 
 
@@ -39,6 +45,7 @@ def test_fast_form_close_to_oracle(form, maker, steps):
     assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
     for f in ("Z", "U", "V", "HMOI"):
         d = np.max(np.abs(g.get(f) - o.get(f)))
+        print(f"[{f}] max abs error {d:.3e}", flush=True)
         assert d <= TOL[form], (f, d)
 
 
@@ -77,11 +84,14 @@ def test_fast_form_sediment(form):
     o.step(8)
     assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
     for f in ("Z", "U", "V"):
-        assert np.max(np.abs(g.get(f) - o.get(f))) <= TOL[form], f
+        d = np.max(np.abs(g.get(f) - o.get(f)))
+        print(f"[{f}] max abs error {d:.3e}", flush=True)
+        assert d <= TOL[form], (f, d)
     for f in ("FS", "DH"):
         a, b = g.get(f), o.get(f)
         scale = np.abs(b).max() + 1e-300
-        assert np.max(np.abs(a - b)) <= 1e3 * TOL[form] * scale, (f, np.max(np.abs(a - b)), scale)
+        print(f"[{f}] relative error {np.max(np.abs(a - b)) / scale:.3e}", flush=True)
+        assert np.max(np.abs(a - b)) <= REL_TOL[form] * scale, (f, np.max(np.abs(a - b)), scale)
 
 
 @pytest.mark.parametrize("form", [HD_FORM_FP64_FAST, HD_FORM_FP32_REFINE], ids=NAMES.get)
@@ -123,7 +133,9 @@ def test_every_line_solver_close_to_oracle(form, kind, param):
     assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
     for f in ("Z", "U", "V"):
         d = np.max(np.abs(g.get(f) - o.get(f)))
+        print(f"[{f}] max abs error {d:.3e}", flush=True)
         assert d <= TOL[form], (f, d)
     a, b = g.get("FS"), o.get("FS")
     scale = np.abs(b).max() + 1e-300
-    assert np.max(np.abs(a - b)) <= 1e3 * TOL[form] * scale
+    print(f"[FS] relative error {np.max(np.abs(a - b)) / scale:.3e}", flush=True)
+    assert np.max(np.abs(a - b)) <= REL_TOL[form] * scale
