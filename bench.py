@@ -354,7 +354,8 @@ def serving_full(seconds, rate, max_active=32, hbm_budget=200 << 30, queue_cap=4
             "median_side_completed": float(np.median(side_done)) if side_done else None,
             "max_active": max_active, "hbm_budget": hbm_budget,
             "workload": "meander+islands, side log-U[128,2048], duration log-U[600,21600] s simulated, "
-                        "sediment p=0.5, FP32+refine p=0.5 else FP64 (fast form)"}
+                        "sediment p=0.5, tolerance log-U[1e-12,1e-4] m: FP32+refine at >= 1e-8 (half), "
+                        "else FP64 (fast form)"}
 
 
 def ensemble_measure(n_scen, steps, forms=("plain", "fast", "fp32")):
@@ -391,14 +392,15 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--serving-seconds", type=float, default=10.0,
-                    help="arrival window of the serving measurement (0 = skip)")
+    ap.add_argument("--serving-seconds", type=float, default=0.0,
+                    help="arrival window of the small-request serving measurement (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=20.0)
-    ap.add_argument("--serving-full", type=float, default=0.0,
-                    help="window (s) of the full-size serving measurement, e.g. 600 (0 = skip)")
-    ap.add_argument("--ensemble", type=int, default=0,
-                    help="scenarios of the ensemble measurement, e.g. 256 (0 = skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=30)
+    ap.add_argument("--serving-full", type=float, default=60.0,
+                    help="arrival window (s) of the serving workload at its full request distribution "
+                         "(600 for the configured window; 0 = skip)")
+    ap.add_argument("--ensemble", type=int, default=256,
+                    help="scenarios of the 512x256 ensemble measurement (0 = skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=60)
     ap.add_argument("--sediment-from-start", action="store_true",
                     help="switch sediment and bed change on at t=0 (short profiling runs)")
     args = ap.parse_args()
