@@ -28,11 +28,19 @@
  * (direction, threshold, exponent 2.1), BedLoad's divergence term, and a
  * steady channel in which surface slope balances the friction term
  * (Ky1 = g hbar^(-2 mu_mn) n^2) and continuity holds.
- * Parity unpinned: the advection and viscosity coefficients of the
- * momentum rows (zero in every case above), BedLoad's one-sided
- * differences near dry cells and its FS-diffusion term, and the
- * non-cohesive source (source_nchs) have no check beyond the whole-run
- * invariants.
+ * tests/test_oracle_pins.py pins the rest: the momentum rows' implicit
+ * advection, viscosity and friction and their explicit cross terms in
+ * every upwind branch (exact on quadratic fields, with the cross
+ * velocity interpolated to the swept face), the continuity rows' flux
+ * form, continuity at every cell of every solved segment for each end
+ * type (extraction), the FS rows (a Fourier mode's closed-form decay
+ * rate, flux form, advection along and across the line, the dry-cell
+ * face rules), both branches of source_nchs, BedLoad's one-sided
+ * differences beside dry cells and its FS diffusion, and hand cases of
+ * Reset_states_vertical's drying and cascading re-wetting.  Each of 18
+ * mistakes planted in a copy of this file (signs, indices, stencil
+ * weights, dropped terms, exponents, thresholds) fails at least one of
+ * those checks.
  *
  * Parallelism: OpenMP over independent lines (sweeps) and rows (stencils).
  * Every parallel loop writes disjoint data, so results do not depend on
@@ -1105,14 +1113,17 @@ FSCell fs_cell(const Oracle &o, int pos, double t, double s_start) {
  * FSj_extract_solution (:642-651) -> _FSj_extract_solution (:316-349).
  * Keep (Q13): gamav is formed before Zf (so 0.98); the end test uses &&.
  * Define: segments shorter than 3 cells have no FS system. */
-void fs_line_j(Oracle &o, int j, double t, double s_start, LineWork &w) {
+void fs_line_j(Oracle &o, int j, double t, double s_start, LineWork &w, int only_seg = -1, int *sn_out = nullptr) {
     const int W = o.W;
     const hd_coeffs &c = o.c;
     const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
     const double *H = o.H_moi.data(), *FS = o.FS.data(), *t_u = o.t_u.data(), *t_v = o.t_v.data();
     const double *Htdu = o.Htdu.data(), *Htdv = o.Htdv.data(), *Kx = o.Kx.data(), *Ky = o.Ky.data();
-    for (const Seg &s : o.segV[j]) {
+    for (int sidx = 0; sidx < (int)o.segV[j].size(); sidx++) {
+        if (only_seg >= 0 && sidx != only_seg) continue;
+        const Seg &s = o.segV[j][sidx];
         const int first = s.first, last = s.last;
+        if (sn_out) *sn_out = -1;
         if (last < first + 1 || last - first < 2) continue;
         bool ran1, ran2;
         wall_flags(o, 1, j, s, ran1, ran2);
@@ -1151,6 +1162,10 @@ void fs_line_j(Oracle &o, int j, double t, double s_start, LineWork &w) {
             }
         }
         const int sn = last - first - 2;
+        if (only_seg >= 0) {  // inspection only: the rows, nothing solved or written
+            if (sn_out) *sn_out = sn;
+            return;
+        }
         thomas(sn, AA, BB, CC, DD, w.x.data(), w.Ap.data(), w.Bp.data());
         const double *x = w.x.data();
         double *tFS = o.tFS.data();
@@ -1178,14 +1193,17 @@ void fs_line_j(Oracle &o, int j, double t, double s_start, LineWork &w) {
 /* FS sweep along j for row i: Scan_FSi (:604-614) ->
  * _FSi_calculate__mactrix_coeff (:51-164), tridiag, FSi_extract_solution
  * (:615-625) -> _FSi_extract_solution (:167-201). */
-void fs_line_i(Oracle &o, int i, double t, double s_start, LineWork &w) {
+void fs_line_i(Oracle &o, int i, double t, double s_start, LineWork &w, int only_seg = -1, int *sn_out = nullptr) {
     const int W = o.W;
     const hd_coeffs &c = o.c;
     const double dX = c.dX, dY = c.dY, dT = c.dT, H_TINH = c.H_TINH;
     const double *H = o.H_moi.data(), *FS = o.FS.data(), *t_u = o.t_u.data(), *t_v = o.t_v.data();
     const double *Htdu = o.Htdu.data(), *Htdv = o.Htdv.data(), *Kx = o.Kx.data(), *Ky = o.Ky.data();
-    for (const Seg &s : o.segH[i]) {
+    for (int sidx = 0; sidx < (int)o.segH[i].size(); sidx++) {
+        if (only_seg >= 0 && sidx != only_seg) continue;
+        const Seg &s = o.segH[i][sidx];
         const int first = s.first, last = s.last;
+        if (sn_out) *sn_out = -1;
         if (last < first + 1 || last - first < 2) continue;
         bool ran1, ran2;
         wall_flags(o, 0, i, s, ran1, ran2);
@@ -1224,6 +1242,10 @@ void fs_line_i(Oracle &o, int i, double t, double s_start, LineWork &w) {
             }
         }
         const int sn = last - first - 2;
+        if (only_seg >= 0) {  // inspection only: the rows, nothing solved or written
+            if (sn_out) *sn_out = sn;
+            return;
+        }
         thomas(sn, AA, BB, CC, DD, w.x.data(), w.Ap.data(), w.Bp.data());
         double *x = w.x.data();
         double *tFS = o.tFS.data();
@@ -1714,6 +1736,10 @@ int ho_run_stage(ho_ctx *ctx, const char *name) {
     else if (n == "heso_k") heso_k(o);
     else if (n == "calculate_qb") calculate_qb(o);
     else if (n == "bed_load") bed_load(o);
+    else if (n == "uz_sweep") uz_sweep(o, 2, o.M);
+    else if (n == "vz_sweep") vz_sweep(o, 2, o.N);
+    else if (n == "reset_states_vertical") reset_states_vertical(o);
+    else if (n == "find_limits") find_limits(o);
     else return HD_E_INVALID;
     return HD_OK;
 }
@@ -1884,6 +1910,51 @@ int ho_segment_line(const int32_t *line, int end, int32_t *first, int32_t *last,
         if (last) last[k] = segs[k].last;
     }
     return (int)segs.size();
+}
+
+int ho_line_coeffs(ho_ctx *ctx, int dir, int line, int seg, double *out, int cap) {
+    if (!ctx || !out) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    const auto &S = dir == 1 ? o.segV : o.segH;
+    if (line < 0 || line >= (int)S.size() || seg < 0 || seg >= (int)S[line].size()) return HD_E_INVALID;
+    const int len = dir == 1 ? o.N : o.M;
+    if (cap < len + 4) return HD_E_INVALID;
+    LineWork w;
+    w.resize(len);
+    const Seg s = S[line][seg];
+    for (int p = s.first; p <= s.last; p++) {
+        if (dir == 1)
+            uz_preindex(o, p, line, w);
+        else
+            vz_preindex(o, line, p, w);
+    }
+    const std::vector<double> *arr[7] = {&w.f1, &w.f2, &w.f3, &w.f5, &w.a2, &w.c2, &w.d2};
+    for (int a = 0; a < 7; a++)
+        for (int p = 0; p < len + 4; p++) out[(size_t)a * cap + p] = (*arr[a])[p];
+    return s.last;
+}
+
+int ho_fs_line_system(ho_ctx *ctx, int dir, int line, int seg, double t, double s_start, double *AA,
+                      double *BB, double *CC, double *DD, int cap) {
+    if (!ctx) return HD_E_INVALID;
+    Oracle &o = ctx->o;
+    const auto &S = dir == 1 ? o.segV : o.segH;
+    if (line < 0 || line >= (int)S.size() || seg < 0 || seg >= (int)S[line].size()) return HD_E_INVALID;
+    LineWork w;
+    w.resize(dir == 1 ? o.N : o.M);
+    int sn = -1;
+    if (dir == 1)
+        fs_line_j(o, line, t, s_start, w, seg, &sn);
+    else
+        fs_line_i(o, line, t, s_start, w, seg, &sn);
+    if (sn + 1 > cap) return HD_E_INVALID;
+    for (int r = 0; r <= sn; r++) {
+        AA[r] = w.AA[r];
+        BB[r] = w.BB[r];
+        CC[r] = w.CC[r];
+        DD[r] = w.DD[r];
+    }
+    return sn;
 }
 
 int ho_line_system(ho_ctx *ctx, int dir, int line, int seg, double *AA, double *BB, double *CC,

@@ -42,7 +42,10 @@ int ho_sediment_i(ho_ctx *ctx, double t);
 int ho_bedload(ho_ctx *ctx, double t);
 /* One kernel restatement on the current state, for closed-form checks:
  * "normalize_u", "normalize_v", "find_vth", "heso_k", "calculate_qb",
- * "bed_load". */
+ * "bed_load", "uz_sweep" / "vz_sweep" (the implicit sweep of every line,
+ * extraction included), "reset_states_vertical" and "find_limits" (the
+ * segment tables from the current khouot, with the FS boundary
+ * interpolation that runs beside it). */
 int ho_run_stage(ho_ctx *ctx, const char *name);
 int ho_step(ho_ctx *o, double *t);
 int ho_run(ho_ctx *o, double *t, double Tmax);
@@ -80,6 +83,21 @@ int ho_segment_line(const int32_t *line, int end, int32_t *first, int32_t *last,
  * (capacity cap) and returns sn (or a negative status). */
 int ho_line_system(ho_ctx *o, int dir, int line, int seg, double *AA, double *BB, double *CC,
                    double *DD, double *x, int cap);
+
+/* Per-position coefficients of one hydraulic segment, as the preindex
+ * kernels form them (_vzSolver_calculate_preindex,
+ * UVZSolver_multithread.cpp:142-208, dir 0; _uzSolver_calculate_preindex
+ * :210-278, dir 1): out[a * cap + p] for a = f1, f2, f3, f5, a2, c2, d2 and
+ * p the position along the line (cap >= line length + 4; positions outside
+ * the segment are 0).  Returns the segment's last position. */
+int ho_line_coeffs(ho_ctx *o, int dir, int line, int seg, double *out, int cap);
+
+/* The FS rows of one segment (dir 1: _FSj_calculate__mactrix_coeff,
+ * sediment_transport.cpp:204-313; dir 0: _FSi_, :51-164) at time t, row k
+ * for position first + 1 + k; nothing is solved.  Returns sn (-1 for a
+ * segment of fewer than 3 cells). */
+int ho_fs_line_system(ho_ctx *o, int dir, int line, int seg, double t, double s_start, double *AA,
+                      double *BB, double *CC, double *DD, int cap);
 
 /* source_chs / source_nchs (sediment_transport.cpp:11-49). */
 double ho_source_chs(const hd_coeffs *c, double wsm, double Zf, double fs, double fw, double vth,

@@ -210,6 +210,29 @@ class Oracle:
         lib().ho_mass_balance(self.h, C.byref(v), C.byref(q))
         return v.value, q.value
 
+    def line_coeffs(self, direction, line, seg):
+        """{f1, f2, f3, f5, a2, c2, d2: array over line positions} of one
+        hydraulic segment (ho_line_coeffs) and the segment's last position."""
+        L = lib()
+        L.ho_line_coeffs.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_int]
+        cap = max(self.scen.M, self.scen.N) + 8
+        out = np.zeros(7 * cap)
+        last = L.ho_line_coeffs(self.h, direction, line, seg, _dp(out), cap)
+        assert last >= 0, last
+        out = out.reshape(7, cap)
+        return {k: out[a] for a, k in enumerate(("f1", "f2", "f3", "f5", "a2", "c2", "d2"))}, last
+
+    def fs_line_system(self, direction, line, seg, t, s_start):
+        """(sn, [AA, BB, CC, DD]) of one FS segment (ho_fs_line_system)."""
+        L = lib()
+        L.ho_fs_line_system.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double] + \
+            [C.POINTER(C.c_double)] * 4 + [C.c_int]
+        cap = max(self.scen.M, self.scen.N) + 8
+        arrs = [np.zeros(cap) for _ in range(4)]
+        sn = L.ho_fs_line_system(self.h, direction, line, seg, t, s_start, *[_dp(a) for a in arrs], cap)
+        assert sn >= -1, sn
+        return sn, [a[:sn + 1] for a in arrs]
+
     def line_system(self, direction, line, seg):
         cap = 2 * max(self.scen.M, self.scen.N) + 8
         arrs = [np.zeros(cap) for _ in range(5)]
