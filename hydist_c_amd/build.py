@@ -16,7 +16,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhydist_amd.so")
 
 SOURCES = ["hd_kernels.hip", "hd_sweep.hip", "hd_api.hip", "hd_context.hip", "hd_server.hip",
-           "hd_compat.hip"]
+           "hd_compat.hip", "hd_gtsv.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

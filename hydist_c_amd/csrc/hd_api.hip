@@ -125,6 +125,8 @@ int set_clock(hd_ctx *ctx, double t) {
     return HD_OK;
 }
 
+int set_library_error(int code, const std::string &msg) { return set_err(nullptr, code, msg); }
+
 int check_launch(hd_ctx *ctx, const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_err(ctx, HD_E_HIP, std::string(what) + ": " + hipGetErrorString(e));

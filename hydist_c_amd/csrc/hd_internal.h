@@ -68,6 +68,9 @@ double *field_ptr(const DevState &s, int f);
 Launch2D grid2d(const DevState &s, int nscen);
 int sync_desc(hd_ctx *ctx);
 int check_launch(hd_ctx *ctx, const char *what);
+// Records msg as the library's last error (hd_last_error(NULL)) and
+// returns code.
+int set_library_error(int code, const std::string &msg);
 void drop_graphs(hd_ctx *ctx);
 hipEvent_t prof_begin(hd_ctx *ctx);
 void prof_end(hd_ctx *ctx, const char *what, hipEvent_t e0);

@@ -67,6 +67,7 @@ def lib():
         L.hd_restore.argtypes = [vp, C.c_void_p, C.c_size_t, dp]
         L.hd_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_double]
         L.hd_load_checkpoint.argtypes = [vp, C.c_char_p, dp]
+        L.hd_gtsv_long.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp]
         L.hd_set_watchdog.argtypes = [vp, C.c_int]
         L.hd_inject_fault.argtypes = [vp, C.c_int]
         _lib = L
@@ -271,6 +272,12 @@ def set_solver(sim, direction, hydraulic, kind, param=0):
 def gtsv_batch(dl, d, du, b, x, m, nsys, ld, precision=0, stream=None):
     """hd_gtsv_batch on device pointers (ints); raises on error."""
     check(lib().hd_gtsv_batch(dl, d, du, b, x, m, nsys, ld, precision, stream), None)
+
+
+def gtsv_long(dl, d, du, b, x, m, precision=0, stream=None):
+    """hd_gtsv_long (one system of any length) on device pointers (ints);
+    raises on error."""
+    check(lib().hd_gtsv_long(dl, d, du, b, x, m, precision, stream), None)
 
 
 def _stage_api():

@@ -323,6 +323,22 @@ int hd_set_solver_threads(hd_ctx *ctx, int dir, int hydraulic, int threads);
 int hd_gtsv_batch(const double *dl, const double *d, const double *du, const double *b, double *x,
                   int m, int nsys, int ld, int precision, void *stream);
 
+/* One tridiagonal system of any length on device buffers: the drop-in for
+ * gtsv_spike_partial_diag_pivot_v1 (gtsv.cpp:88-193), which splits one
+ * system of m rows over many workgroups and solves their interface system
+ * in one block, allocating its work buffers on every call (:111-147).
+ * Same argument meaning as hd_gtsv_batch for nsys = 1 (dl[0], du[m-1]
+ * ignored), 1 <= m <= 4096 * 4096.  Above 4096 rows the system is cut
+ * into chunks of 4096 rows, each chunk's interior is solved by the
+ * partition solver for the right-hand side and two spikes, and the
+ * interface unknowns (one per chunk) by one more partition solve.  Work
+ * memory (4 m + 5 m / 4096 doubles) is allocated and freed on `stream`
+ * for each call.  Synchronises the stream before returning.  Diagonally
+ * dominant systems, as hd_gtsv_batch. */
+int hd_gtsv_long(const double *dl, const double *d, const double *du, const double *b, double *x,
+                 int m, int precision, void *stream);
+
+
 /* Checkpoint / restart.  Replaces the reference's stubs (the -c/-T
  * options, main.cpp:283-284, loader.cpp:172-185): the snapshot holds every
  * buffer whose contents carry from one step to the next (state, derived

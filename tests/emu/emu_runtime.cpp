@@ -228,6 +228,19 @@ hipError_t hipFree(void *p) {
     g_allocs.erase(it);
     return hipSuccess;
 }
+hipError_t hipMallocAsync(void **p, size_t n, hipStream_t) { return hipMalloc(p, n); }
+hipError_t hipFreeAsync(void *p, hipStream_t) { return hipFree(p); }
+hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+void *emu_symbol_buffer(const void *sym, size_t n) {
+    static std::map<const void *, void *> bufs;
+    auto it = bufs.find(sym);
+    if (it != bufs.end()) return it->second;
+    void *p = nullptr;
+    if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+    std::memset(p, 0, n);
+    bufs[sym] = p;
+    return p;
+}
 // "Device" buffers for tests that call entry points taking device pointers.
 extern "C" void *hd_emu_alloc(size_t n) {
     void *p = nullptr;

@@ -24,6 +24,7 @@
 #define __shared__ static
 #define __restrict__ __restrict
 #define HIP_SYMBOL(x) (&(x))
+#define __align__(n) alignas(n)
 
 struct dim3 {
     unsigned x = 1, y = 1, z = 1;
@@ -83,6 +84,18 @@ hipError_t hipMalloc(void **p, size_t n);
 template <class T>
 inline hipError_t hipMalloc(T **p, size_t n) { return hipMalloc((void **)p, n); }
 hipError_t hipFree(void *p);
+hipError_t hipMallocAsync(void **p, size_t n, hipStream_t s);
+hipError_t hipFreeAsync(void *p, hipStream_t s);
+hipError_t hipGetDevice(int *d);
+// A device symbol's address: a guarded buffer of the symbol's size, one per
+// symbol (kernels reach the symbol only through the pointer returned here,
+// so accesses past it fault like those past any device buffer).
+void *emu_symbol_buffer(const void *sym, size_t n);
+template <class T>
+inline hipError_t hipGetSymbolAddress(void **p, T *sym) {
+    *p = emu_symbol_buffer((const void *)sym, sizeof(T));
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
 hipError_t hipMemset(void *p, int v, size_t n);
 hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t s = nullptr);
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k);

@@ -1,7 +1,8 @@
 """Batched tridiagonal solve: hd_gtsv_batch (the drop-in for
 gtsv_spike_partial_diag_pivot_v1, gtsv.cpp:88) and the partition solver
-behind it, against LAPACK's banded solve (scipy.linalg.solve_banded, with
-partial pivoting) in FP64.
+behind it, and hd_gtsv_long (one system of any length, chunked, with an
+interface system), against LAPACK's banded solve
+(scipy.linalg.solve_banded, with partial pivoting) in FP64.
 
 The systems are diagonally dominant and carry zero couplings and identity
 rows, as the lines of the sweeps do.  Rows past m hold NaN and dl[0],
@@ -131,4 +132,40 @@ def test_gtsv_batch_rejects_bad_shapes():
         with pytest.raises(engine.HydistError):
             engine.gtsv_batch(p, p, p, p, p, m, nsys, ld, prec)
     engine.gtsv_batch(p, p, p, p, p, 8, 0, 8, 0)  # no systems: nothing to do
+    one.free()
+
+
+LONG_M = [4097, 5000, 8192, 8193, 12289] + ([] if EMU else [100_003, 1_000_000])
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32r"])
+@pytest.mark.parametrize("m", LONG_M)
+def test_gtsv_long_matches_lapack(m, precision):
+    """Lengths past one chunk (4096 rows): a remainder of one row (8193),
+    of many (5000, 100003), none (8192) and a single-row last chunk
+    (12289 = 3 * 4096 + 1)."""
+    print(f"[case] gtsv_long m={m} precision={precision}", flush=True)
+    rng = np.random.default_rng(7000 + m)
+    dl, d, du, b = make_systems(rng, m, 1, m + 5)
+    ref = reference(dl, d, du, b, m)[0]
+    bufs = [Dev(a[0]) for a in (dl, d, du, b)]
+    xb = Dev(np.full(m + 5, 7.0))
+    engine.gtsv_long(*[q.ptr for q in bufs], xb.ptr, m, precision)
+    x = xb.get()
+    for q in bufs + [xb]:
+        q.free()
+    assert np.all(x[m:] == 7.0), "wrote past m"
+    err = np.max(np.abs(x[:m] - ref))
+    tol = (1e-12 if precision == 0 else 1e-9) * np.max(np.abs(ref))
+    print(f"[gtsv_long] m={m} p={precision} max err {err:.3e} (tol {tol:.1e})", flush=True)
+    assert err <= tol, (m, precision, err)
+
+
+def test_gtsv_long_rejects_bad_shapes():
+    print("[case] gtsv_long shapes", flush=True)
+    one = Dev(np.zeros(8))
+    p = one.ptr
+    for m, prec in ((0, 0), (4096 * 4096 + 1, 0), (8, 2)):
+        with pytest.raises(engine.HydistError):
+            engine.gtsv_long(p, p, p, p, p, m, prec)
     one.free()
