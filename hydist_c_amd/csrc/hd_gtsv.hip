@@ -16,11 +16,14 @@
 // the caller's coefficient arrays in place (chunk g's interior starts at
 // row g * GT_CHUNK, so the chunks are systems GT_CHUNK elements apart).
 //
-// Device memory beyond the caller's arrays: the two spike right-hand sides
-// and solutions (4 m doubles) and the interface system (5 values per
-// chunk), allocated on the caller's stream for each call.
+// Device memory beyond the caller's arrays ("scratch"): the two spike
+// right-hand sides and solutions (4 m doubles) and the interface system
+// (5 values per chunk).  hd_gtsv_long allocates it on the caller's stream
+// for each call; hd_gtsv_long_noalloc takes it from a fixed device array
+// sized for HD_GTSV_NOALLOC_MAX_M rows.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "hd_internal.h"
@@ -144,6 +147,40 @@ int gtsv_long(hd_ctx *ctx, const double *dl, const double *d, const double *du, 
 
 namespace {
 
+// Scratch of hd_gtsv_long_noalloc: one copy per device, held until the
+// process ends.  Host code never uses the array's name as an address: each
+// device's copy is looked up once (hipGetSymbolAddress, with that device
+// current) and the pointer kept per device.
+constexpr size_t kNoAllocDoubles = 4 * (size_t)HD_GTSV_NOALLOC_MAX_M +
+                                   5 * (size_t)((HD_GTSV_NOALLOC_MAX_M + hd::GT_CHUNK - 1) / hd::GT_CHUNK);
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch`, looked up on first use.
+int device_scratch(double **out, std::string &err) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return HD_E_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return HD_E_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return HD_OK;
+}
+
 int gtsv_args(const double *dl, const double *d, const double *du, const double *b, double *x, int m, int precision,
               const char *who, std::string &err) {
     if (!dl || !d || !du || !b || !x) {
@@ -179,6 +216,30 @@ int hd_gtsv_long(const double *dl, const double *d, const double *du, const doub
     if (p) (void)hipFreeAsync(p, c.stream);
     if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) {
         c.err = std::string("hd_gtsv_long: ") + hipGetErrorString(hipGetLastError());
+        rc = HD_E_HIP;
+    }
+    if (rc) return hd::set_library_error(rc, c.err.c_str());
+    return HD_OK;
+}
+
+int hd_gtsv_long_noalloc(const double *dl, const double *d, const double *du, const double *b, double *x, int m,
+                         int precision, void *stream) {
+    hd_ctx c;  // launch settings and error text only; owns nothing
+    int rc = gtsv_args(dl, d, du, b, x, m, precision, "hd_gtsv_long_noalloc", c.err);
+    if (!rc && m > HD_GTSV_NOALLOC_MAX_M) {
+        c.err = "hd_gtsv_long_noalloc: m = " + std::to_string(m) + " exceeds the fixed scratch's " +
+                std::to_string(HD_GTSV_NOALLOC_MAX_M) + " rows (use hd_gtsv_long)";
+        rc = HD_E_INVALID;
+    }
+    if (rc) return hd::set_library_error(rc, c.err.c_str());
+    double *p = nullptr;
+    rc = device_scratch(&p, c.err);
+    if (rc) return hd::set_library_error(rc, c.err.c_str());
+    c.stream = (hipStream_t)stream;
+    c.cfg.form = precision ? HD_FORM_FP32_REFINE : HD_FORM_FP64_FAST;
+    rc = hd::gtsv_long(&c, dl, d, du, b, x, m, p);
+    if (!rc && hipStreamSynchronize(c.stream) != hipSuccess) {
+        c.err = std::string("hd_gtsv_long_noalloc: ") + hipGetErrorString(hipGetLastError());
         rc = HD_E_HIP;
     }
     if (rc) return hd::set_library_error(rc, c.err.c_str());

@@ -68,6 +68,7 @@ def lib():
         L.hd_save_checkpoint.argtypes = [vp, C.c_char_p, C.c_double]
         L.hd_load_checkpoint.argtypes = [vp, C.c_char_p, dp]
         L.hd_gtsv_long.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp]
+        L.hd_gtsv_long_noalloc.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, vp]
         L.hd_set_watchdog.argtypes = [vp, C.c_int]
         L.hd_inject_fault.argtypes = [vp, C.c_int]
         _lib = L
@@ -278,6 +279,15 @@ def gtsv_long(dl, d, du, b, x, m, precision=0, stream=None):
     """hd_gtsv_long (one system of any length) on device pointers (ints);
     raises on error."""
     check(lib().hd_gtsv_long(dl, d, du, b, x, m, precision, stream), None)
+
+
+def gtsv_long_noalloc(dl, d, du, b, x, m, precision=0, stream=None):
+    """hd_gtsv_long_noalloc: as gtsv_long with the library's fixed work
+    array (m <= GTSV_NOALLOC_MAX_M), no device allocation."""
+    check(lib().hd_gtsv_long_noalloc(dl, d, du, b, x, m, precision, stream), None)
+
+
+GTSV_NOALLOC_MAX_M = 262144
 
 
 def _stage_api():

@@ -338,6 +338,19 @@ int hd_gtsv_batch(const double *dl, const double *d, const double *du, const dou
 int hd_gtsv_long(const double *dl, const double *d, const double *du, const double *b, double *x,
                  int m, int precision, void *stream);
 
+/* hd_gtsv_long without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops that must not call hipMalloc, harnesses
+ * that forbid it).  Same arguments and results (bit for bit) as
+ * hd_gtsv_long, for m <= HD_GTSV_NOALLOC_MAX_M; larger m is rejected with
+ * HD_E_INVALID.  Its work memory is a fixed device array of
+ * 4 HD_GTSV_NOALLOC_MAX_M + 320 doubles (8.4 MB) in the library, one per
+ * device, held until the process ends and shared by every call on that
+ * device: calls that overlap on one device (other streams, other host
+ * threads) race on it and must be serialised by the caller. */
+#define HD_GTSV_NOALLOC_MAX_M 262144
+int hd_gtsv_long_noalloc(const double *dl, const double *d, const double *du, const double *b, double *x,
+                         int m, int precision, void *stream);
 
 /* Checkpoint / restart.  Replaces the reference's stubs (the -c/-T
  * options, main.cpp:283-284, loader.cpp:172-185): the snapshot holds every
