@@ -928,7 +928,7 @@ int hd_memory_bytes(hd_ctx *ctx, size_t *bytes) {
 namespace {
 
 struct SnapHeader {
-    char magic[8];  // "HDSNAP1"
+    char magic[8];  // "HDSNAP2"
     int32_t M, N, P, H, capH, capV, nscen, nbuf;
     double t;
     uint64_t bytes;
@@ -944,6 +944,8 @@ std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
                           s.Qbx, s.Qby, s.dH, s.htaiz, s.htaiz_bd, s.Kx1, s.Ky1})
             b.push_back({f, fb});
         b.push_back({s.khouot, cells * sizeof(int)});
+        b.push_back({s.segidH, cells * sizeof(int)});
+        b.push_back({s.segidV, cells * sizeof(int)});
         b.push_back({s.hi, 2 * (size_t)(s.M + s.N + 6) * sizeof(double)});
         b.push_back({s.hmax, 4 * sizeof(double)});
         b.push_back({s.ubt, (s.M + 3) * sizeof(double)});
@@ -963,7 +965,7 @@ std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
 
 SnapHeader snap_header(hd_ctx *ctx, double t, size_t nbuf, size_t bytes) {
     SnapHeader h{};
-    std::memcpy(h.magic, "HDSNAP1", 8);
+    std::memcpy(h.magic, "HDSNAP2", 8);
     const DevState &s = ctx->hs;
     h.M = s.M; h.N = s.N; h.P = s.P; h.H = s.H; h.capH = s.capH; h.capV = s.capV;
     h.nscen = ctx->nscen;

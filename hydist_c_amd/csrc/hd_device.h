@@ -40,6 +40,10 @@ struct DevState {
     // wet-segment tables: rows (along j) and columns (along i)
     int *segH_n, *segH_f, *segH_l;
     int *segV_n, *segV_f, *segV_l;
+    // per cell (layout of the fields): index of the row segment (segidH)
+    // and of the column segment (segidV) containing the cell, -1 outside
+    // every segment; written with the tables by the segmentation kernel
+    int *segidH, *segidV;
     // scratch
     double *tmp;       // smoothing output (full field)
     double *Ap, *Bp;   // Thomas sweep scratch, two rows per cell
@@ -322,25 +326,6 @@ HD_DEV FSCell fs_cell(const DevState &s, int pos, double t, double s_start) {
         r.S = source_nchs(c, r.wsm, r.Zf, Uf, H, fsv);
     if (t < s_start) r.S = 0;
     return r;
-}
-
-// ---------------------------------------------------------------------------
-// Segment lookup in a line's sorted segment list: index of the segment
-// containing position p, or -1.
-// ---------------------------------------------------------------------------
-HD_DEV int find_segment(const int *__restrict__ first, const int *__restrict__ last, int n, int p) {
-    int lo = 0, hi = n - 1, k = -1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        if (first[mid] <= p) {
-            k = mid;
-            lo = mid + 1;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    if (k >= 0 && p <= last[k]) return k;
-    return -1;
 }
 
 // Wall tests of locate_segment_u / locate_segment_v

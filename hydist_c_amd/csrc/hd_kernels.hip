@@ -39,6 +39,8 @@ __global__ void k_onetime_init(const DevState *__restrict__ S) {
         }
     }
     s.khouot[i * W + j] = kh;
+    s.segidH[i * W + j] = -1;
+    s.segidV[i * W + j] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -91,6 +93,14 @@ HD_DEV void segment_rows_block(const DevState &s, int b) {
             prevbit = w[u] >> 63;
             if ((bs >> lane) & 1ull) first[nstart + __popcll(bs & below)] = p;
             if ((be >> lane) & 1ull) last[nend + __popcll(be & below)] = p;
+            if (p <= end) {
+                // segment of p: the starts at or before p; its run starts at the
+                // last of them, and a run starting at end-1 or later is no segment
+                const uint64_t le = bs & (below | (1ull << lane));
+                const int start = le ? base + 63 - __clzll(le) : last_start;
+                const bool in = ((w[u] >> lane) & 1ull) && start < end - 1;
+                s.segidH[i * W + p] = in ? nstart + __popcll(le) - 1 : -1;
+            }
             nstart += __popcll(bs);
             nend += __popcll(be);
             if (bs) last_start = base + 63 - __clzll(bs);
@@ -147,8 +157,14 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
     __syncthreads();
     int off = 0;
     for (int q = 0; q < g; q++) off += cnt[q][tx];
+    // the column's last run start: a run starting at end-1 or later is no
+    // segment, and it is always the last run
+    int lst_all = -1;
+    for (int q = 0; q < SEG_GROUPS; q++) lst_all = max(lst_all, lastst[q][tx]);
+    const int dropped_from = lst_all >= end - 1 ? lst_all : end + 1;
     if (live && r0 < r1) {
         int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
+        int *segid = s.segidV + j;
         int k = off;  // starts at rows <= p
         bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
         for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
@@ -163,6 +179,7 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
                     const bool nwet = p < end && v[u + 1] == 0;
                     if (wet && !prev) first[k++] = p;
                     if (wet && !nwet) last[k - 1] = p;
+                    segid[(size_t)p * W] = (wet && p < dropped_from) ? k - 1 : -1;
                     prev = wet;
                 }
             }
@@ -784,11 +801,9 @@ __global__ void k_bedload(const DevState *__restrict__ S) {
     if (UPDATE_FS && i >= 1 && j >= 1 && i <= s.N && j <= s.M) s.FS[i * s.P + j] = s.tFS[i * s.P + j];
     if (i < 3 || i > s.N - 2 || j < 2 || j > s.M) return;
     const int W = s.P, pos = i * W + j, M = s.M, N = s.N;
-    const int n = s.segH_n[i];
-    const int *fs = s.segH_f + (size_t)i * s.capH, *ls = s.segH_l + (size_t)i * s.capH;
-    const int k = find_segment(fs, ls, n, j);
+    const int k = s.segidH[pos];
     if (k < 0) return;
-    const int first = fs[k], last = ls[k];
+    const int first = s.segH_f[(size_t)i * s.capH + k], last = s.segH_l[(size_t)i * s.capH + k];
     if (last - first < 2) return;
     const hd_coeffs &c = s.c;
     const double dX = c.dX, dY = c.dY, dT = c.dT;

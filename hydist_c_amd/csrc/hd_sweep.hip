@@ -40,16 +40,15 @@ struct SegCtx {
     int rho0;  // first row of the system (-1 when sn < 0)
 };
 
+// Segment k = [f, l] of line `line` containing position p (the per-cell
+// segment index written by the segmentation kernel), or false.
 template <int DIR>
 HD_DEV bool locate(const DevState &s, int line, int p, int &k, int &f, int &l) {
-    const int n = DIR == 1 ? s.segV_n[line] : s.segH_n[line];
-    const int cap = DIR == 1 ? s.capV : s.capH;
-    const int *fs = (DIR == 1 ? s.segV_f : s.segH_f) + (size_t)line * cap;
-    const int *ls = (DIR == 1 ? s.segV_l : s.segH_l) + (size_t)line * cap;
-    k = find_segment(fs, ls, n, p);
+    k = DIR == 1 ? s.segidV[(size_t)p * s.P + line] : s.segidH[(size_t)line * s.P + p];
     if (k < 0) return false;
-    f = fs[k];
-    l = ls[k];
+    const size_t o = (size_t)line * (DIR == 1 ? s.capV : s.capH) + k;
+    f = (DIR == 1 ? s.segV_f : s.segH_f)[o];
+    l = (DIR == 1 ? s.segV_l : s.segH_l)[o];
     return true;
 }
 
