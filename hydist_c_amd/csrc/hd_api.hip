@@ -100,13 +100,23 @@ Launch2D grid2d(const DevState &s, int nscen) {
 // Pushes the host descriptor to the device copy.
 // Pushes the host descriptors to the device copy.  Scenario 0 (hs) holds
 // the parameters every scenario shares (coefficients, quirks, cohesion).
+static void derive_constants(DevState &d) {
+    d.r_dXbp = 1.0 / d.c.dXbp;
+    d.r_dYbp = 1.0 / d.c.dYbp;
+    d.r_dX2 = 1.0 / d.c.dX2;
+    d.r_dY2 = 1.0 / d.c.dY2;
+    d.Dxr_m03 = std::pow(d.c.Dxr, -0.3);
+}
+
 int sync_desc(hd_ctx *ctx) {
     if (!ctx->ds) return HD_OK;
+    derive_constants(ctx->hs);
     for (DevState &d : ctx->hss) {
         d.c = ctx->hs.c;
         d.quirks = ctx->hs.quirks;
         d.ketdinh = ctx->hs.ketdinh;
         d.clock = ctx->hs.clock;
+        derive_constants(d);
     }
     HD_HIP(ctx, hipMemcpyAsync(ctx->ds, &ctx->hs, sizeof(DevState), hipMemcpyHostToDevice, ctx->stream));
     if (!ctx->hss.empty())
@@ -946,6 +956,7 @@ std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
         b.push_back({s.khouot, cells * sizeof(int)});
         b.push_back({s.segidH, cells * sizeof(int)});
         b.push_back({s.segidV, cells * sizeof(int)});
+        b.push_back({s.segflV, 2 * cells * sizeof(int)});
         b.push_back({s.hi, 2 * (size_t)(s.M + s.N + 6) * sizeof(double)});
         b.push_back({s.hmax, 4 * sizeof(double)});
         b.push_back({s.ubt, (s.M + 3) * sizeof(double)});

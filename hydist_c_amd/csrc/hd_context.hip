@@ -63,6 +63,7 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     A((void **)&s.khouot, cells * sizeof(int));
     A((void **)&s.segidH, cells * sizeof(int));
     A((void **)&s.segidV, cells * sizeof(int));
+    A((void **)&s.segflV, 2 * cells * sizeof(int));
     A((void **)&s.hi, 2 * (size_t)(s.M + s.N + 6) * sizeof(double));
     A((void **)&s.hmax, 4 * sizeof(double));
     A((void **)&s.ubt, (s.M + 3) * sizeof(double));
@@ -154,7 +155,7 @@ size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
     const size_t ldT = round_up(N + 3, 32), capH = (M + 3) / 2 + 2, capV = (N + 3) / 2 + 2;
     const size_t cells = P * H;
     auto al = [](size_t b) { return b ? b : (size_t)8; };
-    size_t b = 28 * cells * 8 + 3 * cells * 4;
+    size_t b = 28 * cells * 8 + 5 * cells * 4;
     b += al(2 * (size_t)(M + N + 6) * 8) + 32 + 2 * (M + 3) * 8 + 2 * (N + 3) * 8;
     b += 8 * al((size_t)mesh->total_time * 8);
     b += (N + 3) * 4 + 2 * (N + 3) * capH * 4 + (M + 3) * 4 + 2 * (M + 3) * capV * 4;

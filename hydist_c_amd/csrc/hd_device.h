@@ -27,6 +27,11 @@ struct DevState {
     int hasCC[4];
     int quirks, ketdinh;
     hd_coeffs c;
+    // reciprocals of c's spacing constants (1/dXbp, 1/dYbp, 1/dX2, 1/dY2),
+    // for the fast forms' assembly, which multiplies where the reference
+    // divides; set from c with every descriptor upload
+    double r_dXbp, r_dYbp, r_dX2, r_dY2;
+    double Dxr_m03;  // pow(c.Dxr, -0.3) of the non-cohesive source
     // inputs
     const double *h, *hsnham, *visc, *Fw, *Tsxw, *Tsyw;
     const double *bc[4], *CC[4];
@@ -44,6 +49,11 @@ struct DevState {
     // and of the column segment (segidV) containing the cell, -1 outside
     // every segment; written with the tables by the segmentation kernel
     int *segidH, *segidV;
+    // per cell, two ints: the first and last row of the column segment
+    // containing the cell (-1, -1 outside), so the column kernels, whose
+    // lanes run across columns, read a segment's bounds coalesced instead
+    // of from 64 different lines' tables
+    int *segflV;
     // scratch
     double *tmp;       // smoothing output (full field)
     double *Ap, *Bp;   // Thomas sweep scratch, two rows per cell
@@ -145,15 +155,129 @@ HD_DEV Pre vz_preindex(const DevState &s, int i, int j) {
     return r;
 }
 
+// The fast forms' preindex: the same terms with every division by a
+// spacing constant replaced by a product with its reciprocal and the two
+// divisions by the face depth by one reciprocal (results differ from the
+// reference's by rounding only).
+HD_DEV Pre uz_preindex_fast(const DevState &s, int i, int j) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Htdu = s.Htdu, *Htdv = s.Htdv, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Kx1 = s.Kx1, *Tsxw = s.Tsxw;
+    const double dT = c.dT, dTchia2dY = c.dTchia2dY, dTchia2dX = c.dTchia2dX, H_TINH = c.H_TINH;
+    const double rXbp = s.r_dXbp, rYbp = s.r_dYbp, rY2 = s.r_dY2;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    const int ij = i * W + j;
+    const double uij = u[ij], vis = VIS[ij];
+    const double rH = 1.0 / Htdu[ij];
+    double vtb = (v[i * W + j - 1] + v[ij] + v[(i + 1) * W + j - 1] + v[(i + 1) * W + j]) * 0.25;
+    r.f1 = dTchia2dX * uij + vis * dT * rXbp;
+    r.f2 = -(2 + Kx1[ij] * dT * sqrt(uij * uij + vtb * vtb) * rH + (2 * dT * vis) * rXbp);
+    r.f3 = dT * vis * rXbp - dTchia2dX * uij;
+    if (H_moi[ij - 1] <= H_TINH) {
+        if (vtb < 0) {
+            p = vtb * (-3 * uij + 4 * u[ij + 1] - u[ij + 2]) * rY2;
+            q = (uij - 2 * u[ij + 1] + u[ij + 2]) * rYbp;
+        }
+    } else {
+        if (H_moi[ij + 1] <= H_TINH) {
+            if ((H_moi[ij - 2] > H_TINH) && (vtb > 0)) {
+                p = vtb * (3 * uij - 4 * u[ij - 1] + u[ij - 2]) * rY2;
+                q = (uij - 2 * u[ij - 1] + u[ij - 2]) * rYbp;
+            }
+        } else {
+            p = vtb * (u[ij + 1] - u[ij - 1]) * rY2;
+            q = (u[ij + 1] - 2 * uij + u[ij - 1]) * rYbp;
+        }
+    }
+    r.f5 = -2 * uij + dT * p - dT * c.CORIOLIS_FORCE * vtb - dT * vis * q - dT * (c.Windx - Tsxw[ij]) * rH;
+    r.c2 = dTchia2dX * Htdu[ij];
+    r.a2 = -dTchia2dX * Htdu[(i - 1) * W + j];
+    r.d2 = z[ij] - dTchia2dY * (Htdv[ij] * v[ij] - Htdv[ij - 1] * v[ij - 1]);
+    return r;
+}
+
+HD_DEV Pre vz_preindex_fast(const DevState &s, int i, int j) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Htdu = s.Htdu, *Htdv = s.Htdv, *H_moi = s.H_moi;
+    const double *VIS = s.visc, *Ky1 = s.Ky1, *Tsyw = s.Tsyw;
+    const double dT = c.dT, dTchia2dY = c.dTchia2dY, dTchia2dX = c.dTchia2dX, H_TINH = c.H_TINH;
+    const double rXbp = s.r_dXbp, rYbp = s.r_dYbp, rX2 = s.r_dX2;
+    Pre r;
+    double p = 0.0, q = 0.0;
+    const int ij = i * W + j;
+    const double vij = v[ij], vis = VIS[ij];
+    const double rH = 1.0 / Htdv[ij];
+    double utb = (u[(i - 1) * W + j] + u[ij] + u[(i - 1) * W + j + 1] + u[ij + 1]) * 0.25;
+    r.f1 = dTchia2dY * vij + vis * dT * rYbp;
+    r.f2 = -(2 + Ky1[ij] * dT * sqrt(vij * vij + utb * utb) * rH + (2 * dT * vis) * rYbp);
+    r.f3 = dT * vis * rYbp - dTchia2dY * vij;
+    if (H_moi[(i - 1) * W + j] <= H_TINH) {
+        if (utb < 0) {
+            q = utb * (-3 * vij + 4 * v[(i + 1) * W + j] - v[(i + 2) * W + j]) * rX2;
+            p = (vij - 2 * v[(i + 1) * W + j] + v[(i + 2) * W + j]) * rXbp;
+        }
+    } else {
+        if (H_moi[(i + 1) * W + j] <= H_TINH) {
+            if ((H_moi[(i - 2) * W + j] > H_TINH) && (utb > 0)) {
+                q = utb * (3 * vij - 4 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) * rX2;
+                p = (vij - 2 * v[(i - 1) * W + j] + v[(i - 2) * W + j]) * rXbp;
+            }
+        } else {
+            q = utb * (v[(i + 1) * W + j] - v[(i - 1) * W + j]) * rX2;
+            p = (v[(i + 1) * W + j] - 2 * vij + v[(i - 1) * W + j]) * rXbp;
+        }
+    }
+    r.f5 = -2 * vij + dT * q + dT * c.CORIOLIS_FORCE * utb - dT * vis * p - dT * (c.Windy - Tsyw[ij]) * rH;
+    r.c2 = dTchia2dY * Htdv[ij];
+    r.a2 = -dTchia2dY * Htdv[ij - 1];
+    r.d2 = z[ij] - dTchia2dX * (Htdu[ij] * u[ij] - Htdu[(i - 1) * W + j] * u[(i - 1) * W + j]);
+    return r;
+}
+
 // Momentum row after continuity is substituted (_calculate_abcd,
 // UVZSolver_multithread.cpp:280-342) at position p of segment [f, l]:
 // interior formula, ran1 override at f, ran2 override at l-1, or the short
 // form for segments of one or two cells.  `cur` is the cell's
-// coefficients, `nxt` those of p+1 (unused by the short form and the ran2
-// override).
+// coefficients, `nxt` those of p+1 (only its continuity part a2, c2, d2 is
+// read; unused by the short form and the ran2 override).
 struct Mom {
     double a1, b1, c1, d1;
 };
+
+// The fast forms' version: one reciprocal each of cur.a2 and nxt.c2.
+HD_DEV Mom calc_abcd_fast(int p, int f, int l, double f4, bool ran1, bool ran2, const Pre &cur,
+                          const Pre &nxt) {
+    Mom m;
+    if (l - 1 > f) {
+        const double ra = 1.0 / cur.a2, rc = 1.0 / nxt.c2;
+        const double ka = cur.f1 * ra, kc = cur.f3 * rc;
+        if (ran1 && p == f) {
+            m.a1 = f4;
+            m.b1 = cur.f2 - kc * nxt.a2;
+            m.c1 = -f4 - kc;
+            m.d1 = cur.f5 - kc * nxt.d2;
+        } else if (ran2 && p == l - 1) {
+            m.a1 = f4 - ka;
+            m.b1 = cur.f2 - ka * cur.c2;
+            m.c1 = -f4;
+            m.d1 = cur.f5 - ka * cur.d2;
+        } else {
+            m.a1 = f4 - ka;
+            m.c1 = -f4 - kc;
+            m.b1 = cur.f2 - ka * cur.c2 - kc * nxt.a2;
+            m.d1 = cur.f5 - ka * cur.d2 - kc * nxt.d2;
+        }
+    } else {
+        m.a1 = f4;
+        m.b1 = cur.f2;
+        m.d1 = cur.f5;
+        m.c1 = -f4;
+    }
+    return m;
+}
 
 HD_DEV Mom calc_abcd(int p, int f, int l, double f4, bool ran1, bool ran2, const Pre &cur,
                      const Pre &nxt) {
@@ -328,6 +452,47 @@ HD_DEV FSCell fs_cell(const DevState &s, int pos, double t, double s_start) {
     return r;
 }
 
+// The fast forms' cell terms: pow(Pd, 2.5) and pow(Tx, 1.5) as products
+// with a square root, pow(Dxr, -0.3) precomputed, and the remaining
+// divisions by one reciprocal each (rounding-level differences only).
+HD_DEV double source_chs_fast(const hd_coeffs &c, double wsm, double Zf, double fs, double fw, double vth,
+                              double h_moi) {
+    double toee = c.Toe;
+    const double todd = c.Tod;
+    double S = 0;
+    const double tob = c.ro * fw * (vth * vth);
+    if (h_moi > c.ghtoe) toee = toee * (1 + c.hstoe * ((h_moi - c.ghtoe)));
+    if (tob > toee) {
+        S = c.Mbochat / c.ros * (tob - toee) / toee;
+    } else if (tob < todd) {
+        const double Pd = 1 - tob / todd;
+        const double beta = 1 + Zf / (1.25 + 4.75 * (Pd * Pd * sqrt(Pd)));
+        S = -wsm * (beta * fs) * Pd;
+    }
+    return S;
+}
+
+HD_DEV FSCell fs_cell_fast(const DevState &s, int pos, double t, double s_start) {
+    const hd_coeffs &c = s.c;
+    FSCell r;
+    const double H = s.H_moi[pos], fsv = s.FS[pos], vth = s.VTH[pos];
+    const double cc = 18 * log(12 * H / c.Ks);
+    const double Uf = sqrt(c.g) * fabs(vth) / cc;
+    r.wsm = c.wss * pow4(1 - fsv);
+    r.Zf = 0;
+    if (fsv > 0.00001) r.Zf = r.wsm / (0.4 * (Uf + 2 * r.wsm));
+    if (s.ketdinh) {
+        r.S = source_chs_fast(c, r.wsm, r.Zf, fsv, s.Fw[pos], vth, H);
+    } else {
+        const double gamac = 0.434 + 5.975 * r.Zf + 2.888 * r.Zf * r.Zf;
+        const double Tx = fmax(0.0, (Uf * Uf - c.Ufr * c.Ufr) / (c.Ufr * c.Ufr));
+        const double Cac = 0.015 * c.dm * (Tx * sqrt(Tx)) * s.Dxr_m03 / (0.05 * H);
+        r.S = r.wsm * (Cac - gamac * fsv);
+    }
+    if (t < s_start) r.S = 0;
+    return r;
+}
+
 // Wall tests of locate_segment_u / locate_segment_v
 // (support_funcs.cpp:823-826, :845-851).  dir 1: line = column j; 0: row i.
 HD_DEV void wall_flags(const DevState &s, int dir, int line, int first, int last, bool &ran1,
@@ -337,8 +502,10 @@ HD_DEV void wall_flags(const DevState &s, int dir, int line, int first, int last
     const double *h = s.h;
     if (dir == 1) {
         const int j = line;
-        ran1 = (first > 2) || ((first == 2) && ((h[1 * W + j] + h[1 * W + j - 1]) * 0.5 == ND));
-        ran2 = (last < s.N) || ((last == s.N) && ((h[s.N * W + j] + h[s.N * W + j - 1]) * 0.5 == ND));
+        ran1 = first > 2;
+        if (first == 2) ran1 = (h[1 * W + j] + h[1 * W + j - 1]) * 0.5 == ND;
+        ran2 = last < s.N;
+        if (last == s.N) ran2 = (h[s.N * W + j] + h[s.N * W + j - 1]) * 0.5 == ND;
     } else {
         const int i = line;
         ran1 = (first > 2) ||
@@ -401,9 +568,34 @@ HD_DEV SegInfo seg_info(const DevState &s, int line, int f, int l, bool dkQ1, bo
     return g;
 }
 
-template <int DIR>
+template <int DIR, bool FAST = false>
 HD_DEV Pre preindex(const DevState &s, int line, int p) {
+    if constexpr (FAST) return DIR == 1 ? uz_preindex_fast(s, p, line) : vz_preindex_fast(s, line, p);
     return DIR == 1 ? uz_preindex(s, p, line) : vz_preindex(s, line, p);
+}
+
+// The continuity coefficients a2, c2, d2 of a cell alone (the same
+// expressions as in the preindex functions above; f1..f5 are left 0): what
+// the momentum row of the cell before reads from this one, and what the
+// extraction's end formulas read.
+template <int DIR>
+HD_DEV Pre continuity(const DevState &s, int line, int p) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double *u = s.u, *v = s.v, *z = s.z, *Htdu = s.Htdu, *Htdv = s.Htdv;
+    Pre r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (DIR == 1) {
+        const int i = p, j = line, ij = i * W + j;
+        r.c2 = c.dTchia2dX * Htdu[ij];
+        r.a2 = -c.dTchia2dX * Htdu[(i - 1) * W + j];
+        r.d2 = z[ij] - c.dTchia2dY * (Htdv[ij] * v[ij] - Htdv[ij - 1] * v[ij - 1]);
+    } else {
+        const int i = line, j = p, ij = i * W + j;
+        r.c2 = c.dTchia2dY * Htdv[ij];
+        r.a2 = -c.dTchia2dY * Htdv[ij - 1];
+        r.d2 = z[ij] - c.dTchia2dX * (Htdu[ij] * u[ij] - Htdu[(i - 1) * W + j] * u[(i - 1) * W + j]);
+    }
+    return r;
 }
 
 
@@ -482,6 +674,99 @@ HD_DEV void fs_row_i(const DevState &s, int i, int j, int first, int last, bool 
             B = B + Cc;
         else
             D = D - Cc * FS[i * W + last];
+    }
+}
+
+// The fast forms' FS rows: fs_row_j / fs_row_i with fs_cell_fast and the
+// divisions by the depth and the spacing constants replaced by products
+// with reciprocals (rounding-level differences only).
+HD_DEV void fs_row_j_fast(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                          double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    const double gamav = 0.98;  // formed before Zf (sediment_transport.cpp:247)
+    const FSCell fc = fs_cell_fast(s, pos, t, s_start);
+    const double rH = 1.0 / H[pos], rdiff = rH * s.r_dXbp;
+    const double adv = gamav * 0.5 * (t_u[pos] + t_u[pos - W]) * s.r_dX2;
+    const double km = Htdu[pos - W] * Kx[pos - W], kp = Htdu[pos] * Kx[pos];
+    A = -adv - km * rdiff;
+    Cc = adv - kp * rdiff;
+    B = c.HaiChiadT + (kp + km) * rdiff;
+    double p = 0, q = 0;
+    if ((H[pos - 1] > H_TINH) && (H[pos + 2] > H_TINH)) p = (FS[pos + 1] - FS[pos]);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 1] > H_TINH)) q = (FS[pos] - FS[pos - 1]);
+    p = (rH * s.r_dYbp) * (Htdv[pos] * Ky[pos] * p - Htdv[pos - 1] * Ky[pos - 1] * q);
+    if ((H[pos - 2] > H_TINH) && (H[pos + 2] > H_TINH))
+        q = (t_v[pos] + t_v[pos - 1]) * 0.5 * ((FS[pos + 1] - FS[pos - 1]) * s.r_dY2) * gamav;
+    else
+        q = 0;
+    D = FS[pos] * c.HaiChiadT - q + p + fc.S * rH;
+    if (i == first + 1) {
+        if ((ran1) || (t_u[first * W + j] == 0))
+            B += A;
+        else
+            D -= A * FS[first * W + j];
+    }
+    if (i == last - 1) {
+        if ((ran2) && (t_u[last * W + j] == 0))
+            B += Cc;
+        else
+            D -= Cc * FS[last * W + j];
+    }
+}
+
+HD_DEV void fs_row_i_fast(const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2,
+                          double t, double s_start, double &A, double &B, double &Cc, double &D) {
+    const int W = s.P;
+    const hd_coeffs &c = s.c;
+    const double H_TINH = c.H_TINH;
+    const double *H = s.H_moi, *FS = s.FS, *t_u = s.t_u, *t_v = s.t_v;
+    const double *Htdu = s.Htdu, *Htdv = s.Htdv, *Kx = s.Kx, *Ky = s.Ky;
+    const int pos = i * W + j;
+    const FSCell fc = fs_cell_fast(s, pos, t, s_start);
+    const double gamav = 0.98 - 0.198 * fc.Zf + 0.032 * fc.Zf * fc.Zf;
+    const double rH = 1.0 / H[pos], rdiff = rH * s.r_dYbp;
+    const double adv = gamav * 0.5 * (t_v[pos - 1] + t_v[pos]) * s.r_dY2;
+    const double km = Htdv[pos - 1] * Ky[pos - 1], kp = Htdv[pos] * Ky[pos];
+    A = -adv - km * rdiff;
+    Cc = adv - kp * rdiff;
+    B = c.HaiChiadT + (kp + km) * rdiff;
+    double p = 0, q = 0;
+    if ((H[pos + 2 * W] > H_TINH) && (H[pos - W] > H_TINH)) p = (FS[pos + W] - FS[pos]);
+    if (H[pos - 2 * W] > H_TINH && H[pos + W] > H_TINH) q = (FS[pos] - FS[pos - W]);
+    p = (rH * s.r_dXbp) * (Htdu[pos] * Kx[pos] * p - Htdu[pos - W] * Kx[pos - W] * q);
+    if (H[pos + 2 * W] > H_TINH && H[pos - 2 * W] > H_TINH)
+        q = (t_u[pos] + t_u[pos - W]) * 0.5 * ((FS[pos + W] - FS[pos - W]) * s.r_dX2) * gamav;
+    else
+        q = 0;
+    D = FS[pos] * c.HaiChiadT - q + p + fc.S * rH;
+    if (j == first + 1) {
+        if ((ran1) || (t_v[i * W + first] == 0))
+            B = B + A;
+        else
+            D = D - A * FS[i * W + first];
+    }
+    if (j == last - 1) {
+        if ((ran2) || (t_v[i * W + last] == 0))
+            B = B + Cc;
+        else
+            D = D - Cc * FS[i * W + last];
+    }
+}
+
+template <bool FAST>
+HD_DEV void fs_row(int dir, const DevState &s, int i, int j, int first, int last, bool ran1, bool ran2, double t,
+                   double s_start, double &A, double &B, double &Cc, double &D) {
+    if constexpr (FAST) {
+        if (dir == 1) fs_row_j_fast(s, i, j, first, last, ran1, ran2, t, s_start, A, B, Cc, D);
+        else fs_row_i_fast(s, i, j, first, last, ran1, ran2, t, s_start, A, B, Cc, D);
+    } else {
+        if (dir == 1) fs_row_j(s, i, j, first, last, ran1, ran2, t, s_start, A, B, Cc, D);
+        else fs_row_i(s, i, j, first, last, ran1, ran2, t, s_start, A, B, Cc, D);
     }
 }
 

@@ -9,7 +9,7 @@ __global__ void k_onetime_init(const DevState *S);
 // k_segment: workgroups of SEG_COLS x SEG_GROUPS threads; a row block takes
 // SEG_ROWS_PER_BLOCK rows (one per wavefront), a column block SEG_COLS
 // columns
-constexpr int SEG_COLS = 32, SEG_GROUPS = 32;
+constexpr int SEG_COLS = 16, SEG_GROUPS = 64;
 constexpr int SEG_ROWS_PER_BLOCK = 4;
 __global__ void k_segment(const DevState *S, int row_blocks, int col_blocks);
 __global__ void k_htuongdoi(const DevState *S);

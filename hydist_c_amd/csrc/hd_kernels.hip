@@ -41,6 +41,8 @@ __global__ void k_onetime_init(const DevState *__restrict__ S) {
     s.khouot[i * W + j] = kh;
     s.segidH[i * W + j] = -1;
     s.segidV[i * W + j] = -1;
+    s.segflV[2 * (i * W + j)] = -1;
+    s.segflV[2 * (i * W + j) + 1] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -60,7 +62,7 @@ __global__ void k_onetime_init(const DevState *__restrict__ S) {
 // takes rows 2 + b * SEG_ROWS_PER_BLOCK onwards, one per wavefront; its
 // other wavefronts have nothing to do (few rows per block spread the row
 // walks over all CUs).
-constexpr int SEG_TILE = 8;
+constexpr int SEG_TILE = 16;
 
 HD_DEV void segment_rows_block(const DevState &s, int b) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -113,7 +115,7 @@ HD_DEV void segment_rows_block(const DevState &s, int b) {
 }
 
 // Columns (lines along i, stride P): a workgroup takes SEG_COLS adjacent
-// columns (one cache line of khouot per row) and splits the rows 2..N into
+// columns (64 bytes of khouot per row) and splits the rows 2..N into
 // SEG_GROUPS contiguous groups, one thread per (group, column).  Pass 1
 // counts run starts per group, an LDS prefix gives each group its first
 // segment index, pass 2 writes the run starts and ends.  Both passes load
@@ -183,6 +185,30 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
                     prev = wet;
                 }
             }
+        }
+    }
+    // pass 3, once every group's starts and ends are in the tables: each
+    // cell's segment bounds beside it
+    __syncthreads();
+    // (SEG_TILE rows at a time, their loads unconditional and clamped to the
+    // group, so the index and table loads of a batch overlap)
+    if (live && r0 < r1) {
+        const int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
+        for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
+            int kk[SEG_TILE];
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++) kk[u] = s.segidV[(size_t)min(p0 + u, r1 - 1) * W + j];
+            int2 fl[SEG_TILE];
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++) {
+                const int k = max(kk[u], 0);
+                fl[u] = make_int2(first[k], last[k]);
+            }
+#pragma unroll
+            for (int u = 0; u < SEG_TILE; u++)
+                if (p0 + u < r1)
+                    *reinterpret_cast<int2 *>(s.segflV + 2 * ((size_t)(p0 + u) * W + j)) =
+                        kk[u] >= 0 ? fl[u] : make_int2(-1, -1);
         }
     }
     if (g == 0 && live) {

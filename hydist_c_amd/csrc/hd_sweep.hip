@@ -34,21 +34,44 @@ namespace hd {
 // ---------------------------------------------------------------------------
 // Segment description shared by assembly and extraction
 // ---------------------------------------------------------------------------
+// The end values a segment's boundary rows read (the water levels at its
+// first and last cells, the discharge boundary values of its line) are
+// loaded where a row needs them (tz_first / tz_last / ub_start / ub_end),
+// not for every cell.
 struct SegCtx {
     SegInfo g;
-    double TZ_l, TZ_r, ub_start, ub_end;
     int rho0;  // first row of the system (-1 when sn < 0)
 };
 
-// Segment k = [f, l] of line `line` containing position p (the per-cell
-// segment index written by the segmentation kernel), or false.
 template <int DIR>
-HD_DEV bool locate(const DevState &s, int line, int p, int &k, int &f, int &l) {
-    k = DIR == 1 ? s.segidV[(size_t)p * s.P + line] : s.segidH[(size_t)line * s.P + p];
+HD_DEV double tz_at(const DevState &s, int line, int p) {
+    return s.t_z[DIR == 1 ? (size_t)p * s.P + line : (size_t)line * s.P + p];
+}
+template <int DIR>
+HD_DEV double ub_start(const DevState &s, int line) { return DIR == 1 ? s.ubt[line] : s.vbd[line]; }
+template <int DIR>
+HD_DEV double ub_end(const DevState &s, int line) { return DIR == 1 ? s.ubp[line] : s.vbt[line]; }
+
+// Segment [f, l] of line `line` containing position p, or false; k is the
+// segment's index in its line's table (columns: read only when the caller
+// asks, with want_k).
+template <int DIR>
+HD_DEV bool locate(const DevState &s, int line, int p, int &k, int &f, int &l, bool want_k = true) {
+    if (DIR == 1) {
+        const size_t c = (size_t)p * s.P + line;
+        const int2 fl = *reinterpret_cast<const int2 *>(s.segflV + 2 * c);
+        f = fl.x;
+        l = fl.y;
+        k = -1;
+        if (f < 0) return false;
+        if (want_k) k = s.segidV[c];
+        return true;
+    }
+    k = s.segidH[(size_t)line * s.P + p];
     if (k < 0) return false;
-    const size_t o = (size_t)line * (DIR == 1 ? s.capV : s.capH) + k;
-    f = (DIR == 1 ? s.segV_f : s.segH_f)[o];
-    l = (DIR == 1 ? s.segV_l : s.segH_l)[o];
+    const size_t o = (size_t)line * s.capH + k;
+    f = s.segH_f[o];
+    l = s.segH_l[o];
     return true;
 }
 
@@ -59,12 +82,6 @@ HD_DEV SegCtx seg_ctx(const DevState &s, int line, int f, int l) {
     const bool dkQ2 = DIR == 1 ? s.bienQ[3] : s.bienQ[0];
     const bool q11 = (s.quirks & HD_QUIRK_Q11_LITERAL) != 0;
     c.g = seg_info<DIR>(s, line, f, l, dkQ1, dkQ2, q11);
-    const int W = s.P;
-    auto cell = [&](int p) { return DIR == 1 ? p * W + line : line * W + p; };
-    c.TZ_l = s.t_z[cell(f)];
-    c.TZ_r = s.t_z[cell(l)];
-    c.ub_start = DIR == 1 ? s.ubt[line] : s.vbd[line];
-    c.ub_end = DIR == 1 ? s.ubp[line] : s.vbt[line];
     c.rho0 = c.g.sn < 0 ? -1 : 2 * (f - 2) + (c.g.isB ? 0 : 1);
     return c;
 }
@@ -87,12 +104,19 @@ HD_DEV signed char row_code(int loc, int sn) {
     return loc == 0 ? 1 : (loc == sn ? 3 : 2);
 }
 
-// The two rows owned by position p of its segment (see the layout rules in
-// the reference's bienrandau / bienlongdau and end overrides).
-template <int DIR>
-HD_DEV void hyd_rows(const DevState &s, int line, int p, const SegCtx &sc, double f4, Row &rz, Row &ru) {
+// The two rows owned by position p of its segment [f, l] (see the layout
+// rules in the reference's bienrandau / bienlongdau and end overrides); sc
+// receives the segment's context.  The cell's own coefficients are formed
+// first: their loads do not depend on the segment, so they are in flight
+// while the segment's context is read.  FAST: the fast forms' arithmetic
+// (preindex_fast, calc_abcd_fast); otherwise the reference's, bit for bit.
+template <int DIR, bool FAST = false>
+HD_DEV void hyd_rows(const DevState &s, int line, int p, int f, int l, double f4, Row &rz, Row &ru, SegCtx &sc) {
+    const Pre cur = preindex<DIR, FAST>(s, line, p);
+    const Pre nx = continuity<DIR>(s, line, p + 1);  // p + 1 <= L + 1: inside the field
+    sc = seg_ctx<DIR>(s, line, f, l);
     const SegInfo &g = sc.g;
-    const int f = g.f, l = g.l, sn = g.sn;
+    const int sn = g.sn;
     rz = identity_row();
     ru = identity_row();
     if (sn < 0) return;
@@ -101,16 +125,13 @@ HD_DEV void hyd_rows(const DevState &s, int line, int p, const SegCtx &sc, doubl
     const int rz_loc = 2 * p - base, ru_loc = 2 * p + 1 - base;
     const bool hz = rz_loc >= 0 && rz_loc <= sn;
     const bool hu = ru_loc >= 0 && ru_loc <= sn;
-    if (!hz && !hu) return;
-    const Pre cur = preindex<DIR>(s, line, p);
-    Pre nxt = cur;
-    if (p + 1 <= l) nxt = preindex<DIR>(s, line, p + 1);
+    const Pre &nxt = p + 1 <= l ? nx : cur;
     if (hz) {
         // continuity of cell p
         Row r{cur.a2, 1.0, cur.c2, cur.d2};
-        if (rz_loc == 0 && g.isB && !g.ran1) r.d = cur.d2 - cur.a2 * sc.ub_start;
+        if (rz_loc == 0 && g.isB && !g.ran1) r.d = cur.d2 - cur.a2 * ub_start<DIR>(s, line);
         if (rz_loc == sn) {
-            if (g.mat_end == END_Q) r.d = cur.d2 - cur.c2 * sc.ub_end;
+            if (g.mat_end == END_Q) r.d = cur.d2 - cur.c2 * ub_end<DIR>(s, line);
             else r.d = cur.d2;  // END_WALL (END_LONG never ends on a continuity row)
         }
         if (rz_loc == 0) r.a = 0.0;
@@ -118,10 +139,11 @@ HD_DEV void hyd_rows(const DevState &s, int line, int p, const SegCtx &sc, doubl
         rz = r;
     }
     if (hu) {
-        const Mom m = calc_abcd(p, f, l, f4, g.ran1, g.ran2, cur, nxt);
+        const Mom m = FAST ? calc_abcd_fast(p, f, l, f4, g.ran1, g.ran2, cur, nxt)
+                           : calc_abcd(p, f, l, f4, g.ran1, g.ran2, cur, nxt);
         Row r{m.a1, m.b1, m.c1, m.d1};
-        if (ru_loc == 0 && !g.isB) r.d = m.d1 - m.a1 * sc.TZ_l;
-        if (ru_loc == sn) r.d = m.d1 - m.c1 * sc.TZ_r;  // END_LONG
+        if (ru_loc == 0 && !g.isB) r.d = m.d1 - m.a1 * tz_at<DIR>(s, line, f);
+        if (ru_loc == sn) r.d = m.d1 - m.c1 * tz_at<DIR>(s, line, l);  // END_LONG
         if (ru_loc == 0) r.a = 0.0;
         if (ru_loc == sn) r.c = 0.0;
         ru = r;
@@ -152,8 +174,8 @@ __global__ void __launch_bounds__(512) k_hyd_assemble(const DevState *__restrict
     if (active) {
         int k, f, l;
         if (locate<DIR>(s, line, p, k, f, l)) {
-            const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
-            hyd_rows<DIR>(s, line, p, sc, f4, rz, ru);
+            SegCtx sc;
+            hyd_rows<DIR, LAYOUT == LINE_MAJOR>(s, line, p, f, l, f4, rz, ru, sc);
             const int base = sc.g.isB ? 2 * f : 2 * f + 1;  // as in hyd_rows
             cz = row_code(2 * p - base, sc.g.sn);
             cu = row_code(2 * p + 1 - base, sc.g.sn);
@@ -260,9 +282,9 @@ __global__ void __launch_bounds__(512) k_fs_assemble(const DevState *__restrict_
             code = row_code(p - (f + 1), l - f - 2);
             if (p >= f + 1 && p <= l - 1) {
                 if (DIR == 1)
-                    fs_row_j(s, p, line, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                    fs_row<LAYOUT == LINE_MAJOR>(1, s, p, line, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
                 else
-                    fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                    fs_row<LAYOUT == LINE_MAJOR>(0, s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
                 if (p == f + 1) r.a = 0.0;
                 if (p == l - 1) r.c = 0.0;
             }
@@ -342,10 +364,11 @@ __global__ void __launch_bounds__(512) k_hyd_assemble_cols(const DevState *__res
         rz[c] = identity_row();
         ru[c] = identity_row();
         int k, f, l;
-        if (j >= start && j <= end && i <= L && locate<1>(s, j, i, k, f, l)) {
-            const SegCtx sc = seg_ctx<1>(s, j, f, l);
-            hyd_rows<1>(s, j, i, sc, f4, rz[c], ru[c]);
+        if (j >= start && j <= end && i <= L && locate<1>(s, j, i, k, f, l, false)) {
+            SegCtx sc;
+            hyd_rows<1, true>(s, j, i, f, l, f4, rz[c], ru[c], sc);
             if (i == f) {
+                k = s.segidV[(size_t)i * s.P + j];
                 B.sys0[(size_t)j * s.capV + k] = sc.rho0;
                 B.sysn[(size_t)j * s.capV + k] = sc.g.sn;
             }
@@ -398,11 +421,13 @@ __global__ void __launch_bounds__(512) k_fs_assemble_cols(const DevState *__rest
         rw[c] = identity_row();
         if (j < start || j > end || i > L) continue;
         int k, f, l;
-        if (locate<1>(s, j, i, k, f, l) && l - f >= 2) {
+        const bool in = locate<1>(s, j, i, k, f, l, false);
+        if (in && i == f) k = s.segidV[(size_t)i * s.P + j];
+        if (in && l - f >= 2) {
             bool ran1, ran2;
             wall_flags(s, 1, j, f, l, ran1, ran2);
             if (i >= f + 1 && i <= l - 1) {
-                fs_row_j(s, i, j, f, l, ran1, ran2, tm, s_start, rw[c].a, rw[c].b, rw[c].c, rw[c].d);
+                fs_row_j_fast(s, i, j, f, l, ran1, ran2, tm, s_start, rw[c].a, rw[c].b, rw[c].c, rw[c].d);
                 if (i == f + 1) rw[c].a = 0.0;
                 if (i == l - 1) rw[c].c = 0.0;
             }
@@ -410,7 +435,7 @@ __global__ void __launch_bounds__(512) k_fs_assemble_cols(const DevState *__rest
                 B.sys0[(size_t)j * s.capV + k] = f + 1 - 2;
                 B.sysn[(size_t)j * s.capV + k] = l - f - 2;
             }
-        } else if (k >= 0 && i == f) {
+        } else if (in && i == f) {
             B.sys0[(size_t)j * s.capV + k] = -1;
             B.sysn[(size_t)j * s.capV + k] = -1;
         }
@@ -717,7 +742,7 @@ template <int DIR, typename XAt>
 HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
     const int L = DIR == 1 ? s.N : s.M;
     int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l)) return;
+    if (!locate<DIR>(s, line, p, k, f, l, false)) return;
     const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
     const SegInfo &g = sc.g;
     const int W = s.P;
@@ -730,12 +755,12 @@ HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
         if (g.isB) {
             tz[cell(p)] = xat(rz);
             tv[cell(p)] = xat(ru);
-            if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : sc.ub_start;
+            if (p == f) tv[cell(f - 1)] = g.ran1 ? 0.0 : ub_start<DIR>(s, line);
         } else {
             tv[cell(p)] = xat(ru);
             if (p > f) tz[cell(p)] = xat(rz);
             if (p == f) {
-                const Pre e = preindex<DIR>(s, line, f);
+                const Pre e = continuity<DIR>(s, line, f);
                 tv[cell(f - 1)] = (e.d2 - tz[cell(f)] - e.c2 * xat(ru)) / e.a2;
             }
         }
@@ -745,10 +770,10 @@ HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
             tv[cell(l)] = 0;
             if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
         } else if (ext_Q) {
-            tv[cell(l)] = sc.ub_end;
+            tv[cell(l)] = ub_end<DIR>(s, line);
             if (sn >= 0) tz[cell(l)] = xat(sc.rho0 + sn);
         } else {
-            const Pre e = preindex<DIR>(s, line, l);
+            const Pre e = continuity<DIR>(s, line, l);
             const double prev = l > f ? xat(2 * (l - 1 - 2) + 1) : tv[cell(l - 1)];
             tv[cell(l)] = (e.d2 - e.a2 * prev - tz[cell(l)]) / e.c2;
         }
@@ -761,7 +786,7 @@ HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
 template <int DIR, typename XV>
 HD_DEV void fs_extract_cell(const DevState &s, int line, int p, XV xv) {
     int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l) || l - f < 2) return;
+    if (!locate<DIR>(s, line, p, k, f, l, false) || l - f < 2) return;
     if (p < f + 1 || p > l - 1) return;
     bool ran1, ran2;
     wall_flags(s, DIR, line, f, l, ran1, ran2);
@@ -971,7 +996,7 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
                     bool ran1, ran2;
                     wall_flags(s, 0, line, f, l, ran1, ran2);
                     if (p >= f + 1 && p <= l - 1) {
-                        fs_row_i(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
+                        fs_row_i_fast(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
                         if (p == f + 1) r.a = 0.0;
                         if (p == l - 1) r.c = 0.0;
                     }
@@ -999,8 +1024,8 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
             Row rz = identity_row(), ru = identity_row();
             int k, f, l;
             if (p <= s.M && locate<0>(s, line, p, k, f, l)) {
-                const SegCtx sc = seg_ctx<0>(s, line, f, l);
-                hyd_rows<0>(s, line, p, sc, f4, rz, ru);
+                SegCtx sc;
+                hyd_rows<0, true>(s, line, p, f, l, f4, rz, ru, sc);
                 if (p == f) {
                     B.sys0[(size_t)line * s.capH + k] = sc.rho0;
                     B.sysn[(size_t)line * s.capH + k] = sc.g.sn;

@@ -49,6 +49,10 @@ struct alignas(16) double2 {
     }
 };
 inline double2 make_double2(double x, double y) { return double2(x, y); }
+struct alignas(8) int2 {
+    int x, y;
+};
+inline int2 make_int2(int x, int y) { return int2{x, y}; }
 
 typedef int hipError_t;
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNoDevice = 100 };
