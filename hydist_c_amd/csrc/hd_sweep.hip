@@ -590,8 +590,9 @@ __global__ void __launch_bounds__(64) k_thomas(SweepBufs Bin, int start, int end
 // the stored elimination) and added.  The velocities are then recovered in
 // FP64 from their own rows.  FS (!INTERLEAVED): the line's system is
 // solved directly.  cr_solve / cr_replay below are the cyclic reduction of
-// n rows held in LDS that both k_cr_reg (the whole line) and k_part (its
-// reduced system) use.
+// n rows held in LDS that k_cr_reg uses for the whole line; k_part solves
+// its reduced system (one row per thread) by parallel cyclic reduction,
+// pcr_solve / pcr_replay.
 // ---------------------------------------------------------------------------
 template <typename T>
 struct CRShared {
@@ -702,6 +703,100 @@ HD_DEV void cr_replay(CRShared<T> sh, int n) {
     }
 }
 
+// Parallel cyclic reduction of k_part's reduced system: n = blockDim.x
+// rows, row t held by thread t.  Step s eliminates from every row its
+// couplings to the rows h = 2^s away on both sides (rows beyond the ends
+// act as identity rows), so after ceil(log2 n) steps every row stands
+// alone: no back substitution, every thread busy at every step, and the
+// reads of a step are contiguous neighbour rows (no LDS bank conflicts).
+// A step reads the rows from one buffer and writes them to the other, so
+// one barrier per step keeps the reads and writes apart.  The solution
+// goes to x.  With `store`, each step's multipliers and the final pivots
+// are kept so pcr_replay can solve a second right-hand side.
+// LDS layout from `base` (n elements each): buffer u's a, b, c, d at
+// 4u .. 4u + 3, then x, the final pivots and, per step, k1 and k2.  The
+// buffers are addressed by offsets, not by an array of pointers, which
+// would sit in scratch memory.
+template <typename T>
+struct PCRShared {
+    T *base;
+    int n;
+    HD_DEV T *row(int buf, int field) const { return base + (4 * buf + field) * n; }
+    HD_DEV T *x() const { return base + 8 * n; }
+    HD_DEV T *bf() const { return base + 9 * n; }
+    HD_DEV T *k(int step, int which) const { return base + (10 + 2 * step + which) * n; }
+};
+
+// Steps of the reduction of n rows: ceil(log2 n).  LDS elements of T:
+// 10 n, plus 2 n per step when multipliers are kept.
+__host__ __device__ inline int pcr_steps(int n) {
+    int s = 0;
+    while ((1 << s) < n) s++;
+    return s;
+}
+
+// Solves the system whose row t is (a, b, c, d) of thread t; ends with a
+// barrier, after which sh.x() holds the solution.
+template <typename T>
+HD_DEV void pcr_solve(const PCRShared<T> &sh, T a, T b, T c, T d, bool store) {
+    const int i = threadIdx.x, n = sh.n;
+    int cur = 0, s = 0;
+    sh.row(0, 0)[i] = a;
+    sh.row(0, 1)[i] = b;
+    sh.row(0, 2)[i] = c;
+    sh.row(0, 3)[i] = d;
+    __syncthreads();
+    for (int h = 1; h < n; h <<= 1, s++) {
+        const bool hl = i >= h, hr = i + h < n;
+        const int il = hl ? i - h : i, ir = hr ? i + h : i;
+        const T *ra = sh.row(cur, 0), *rb = sh.row(cur, 1), *rc = sh.row(cur, 2), *rd = sh.row(cur, 3);
+        const T al = ra[il], bl = rb[il], cl = rc[il], dl = rd[il];
+        const T ar = ra[ir], br = rb[ir], cr = rc[ir], dr = rd[ir];
+        const T k1 = hl ? a / bl : T(0);
+        const T k2 = hr ? c / br : T(0);
+        b = b - cl * k1 - ar * k2;
+        d = d - dl * k1 - dr * k2;
+        a = -(al * k1);
+        c = -(cr * k2);
+        if (store) {
+            sh.k(s, 0)[i] = k1;
+            sh.k(s, 1)[i] = k2;
+        }
+        if (2 * h >= n) break;  // the last step's rows are read by no one
+        cur ^= 1;
+        sh.row(cur, 0)[i] = a;
+        sh.row(cur, 1)[i] = b;
+        sh.row(cur, 2)[i] = c;
+        sh.row(cur, 3)[i] = d;
+        __syncthreads();
+    }
+    sh.x()[i] = d / b;
+    if (store) sh.bf()[i] = b;
+    __syncthreads();
+}
+
+// Solves the system of the last pcr_solve(store = true) for the right-hand
+// side d of thread t; ends with a barrier, after which sh.x() holds the
+// solution.  Only the d buffers, x and the caller's data change.
+template <typename T>
+HD_DEV void pcr_replay(const PCRShared<T> &sh, T d) {
+    const int i = threadIdx.x, n = sh.n;
+    int cur = 0, s = 0;
+    sh.row(0, 3)[i] = d;
+    __syncthreads();
+    for (int h = 1; h < n; h <<= 1, s++) {
+        const T *rd = sh.row(cur, 3);
+        const T dl = rd[i >= h ? i - h : i], dr = rd[i + h < n ? i + h : i];
+        d = d - dl * sh.k(s, 0)[i] - dr * sh.k(s, 1)[i];
+        if (2 * h >= n) break;
+        cur ^= 1;
+        sh.row(cur, 3)[i] = d;
+        __syncthreads();
+    }
+    sh.x()[i] = d / sh.bf()[i];
+    __syncthreads();
+}
+
 // A z row of an interleaved line with its neighbouring velocity rows
 // eliminated in FP64: zr is row 2q, m1 row 2q-1 (used when has_m1), p1 row
 // 2q+1 (used when has_p1).
@@ -733,16 +828,43 @@ HD_DEV Row reduce_z_row(const double *__restrict__ A, const double *__restrict__
     return reduce_z_vals(m1, zr, p1, has_m1, has_p1);
 }
 
+// Segments of the K positions p = tid + m * nt + 2 of row line `line` that
+// a k_part thread assembles: k (the index in the line's table, -1 where p
+// is in no segment) and the bounds f, l.  All K index loads go out at
+// once, then all K table loads (indices clamped, so both rounds are
+// unconditional), so the positions wait for two memory latencies together
+// instead of two each.
+template <int K>
+HD_DEV void row_segments(const DevState &s, int line, int tid, int nt, int (&kk)[K], int (&ff)[K], int (&ll)[K]) {
+#pragma unroll
+    for (int m = 0; m < K; m++) kk[m] = s.segidH[(size_t)line * s.P + min(tid + m * nt + 2, s.M)];
+#pragma unroll
+    for (int m = 0; m < K; m++) {
+        const size_t o = (size_t)line * s.capH + max(kk[m], 0);
+        ff[m] = s.segH_f[o];
+        ll[m] = s.segH_l[o];
+    }
+#pragma unroll
+    for (int m = 0; m < K; m++)
+        if (kk[m] < 0) ff[m] = ll[m] = 2;  // in-range bounds for the callers' unconditional loads
+}
+
 // ---------------------------------------------------------------------------
 // Extraction (hydraulics).  Define: cells before `last` are written before
 // the end cell, whose continuity formula uses the new velocity at last-1.
 // ---------------------------------------------------------------------------
 // Cell p of line `line`; xat(rho) reads the line's solution row rho.
 template <int DIR, typename XAt>
+HD_DEV void hyd_extract_at(const DevState &s, int line, int p, int f, int l, XAt xat);
+template <int DIR, typename XAt>
 HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
-    const int L = DIR == 1 ? s.N : s.M;
     int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l, false)) return;
+    if (locate<DIR>(s, line, p, k, f, l, false)) hyd_extract_at<DIR>(s, line, p, f, l, xat);
+}
+// The same for a cell whose segment [f, l] is known.
+template <int DIR, typename XAt>
+HD_DEV void hyd_extract_at(const DevState &s, int line, int p, int f, int l, XAt xat) {
+    const int L = DIR == 1 ? s.N : s.M;
     const SegCtx sc = seg_ctx<DIR>(s, line, f, l);
     const SegInfo &g = sc.g;
     const int W = s.P;
@@ -784,10 +906,16 @@ HD_DEV void hyd_extract_cell(const DevState &s, int line, int p, XAt xat) {
 // become NDnen; the segment's end cells take the neighbouring value or keep
 // FS as the reference's end tests decide.  xv() reads the solution at p.
 template <int DIR, typename XV>
+HD_DEV void fs_extract_at(const DevState &s, int line, int p, int f, int l, XV xv);
+template <int DIR, typename XV>
 HD_DEV void fs_extract_cell(const DevState &s, int line, int p, XV xv) {
     int k, f, l;
-    if (!locate<DIR>(s, line, p, k, f, l, false) || l - f < 2) return;
-    if (p < f + 1 || p > l - 1) return;
+    if (locate<DIR>(s, line, p, k, f, l, false)) fs_extract_at<DIR>(s, line, p, f, l, xv);
+}
+// The same for a cell whose segment [f, l] is known.
+template <int DIR, typename XV>
+HD_DEV void fs_extract_at(const DevState &s, int line, int p, int f, int l, XV xv) {
+    if (l - f < 2 || p < f + 1 || p > l - 1) return;
     bool ran1, ran2;
     wall_flags(s, DIR, line, f, l, ran1, ran2);
     const int W = s.P;
@@ -935,7 +1063,7 @@ __global__ void __launch_bounds__(512) k_cr_reg(const DevState *__restrict__ S, 
 // where L_t is the thread's last unknown.  Substituted into the last row,
 // this gives one row per thread in the L: a tridiagonal system (a Schur
 // complement of the line's, so diagonally dominant like it) solved by
-// cyclic reduction in LDS, after which every x_k follows.  Rows pass
+// parallel cyclic reduction in LDS, after which every x_k follows.  Rows pass
 // through LDS on the way in and out so that global accesses stay coalesced
 // while each thread works on a contiguous chunk.  With REFINE the T
 // solution is corrected once by solving for the FP64 residual with the
@@ -970,8 +1098,7 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     T *ev = reinterpret_cast<T *>(smem_part + nt);      // v_0
     T *ew = reinterpret_cast<T *>(smem_part + 2 * nt);  // w_0
     double *ez = smem_part + 3 * nt;                    // x_0 in FP64
-    T *crb = reinterpret_cast<T *>(smem_part + 4 * nt);
-    CRShared<T> sh{crb, crb + nt, crb + 2 * nt, crb + 3 * nt, crb + 4 * nt, crb + 5 * nt};
+    const PCRShared<T> sh{reinterpret_cast<T *>(smem_part + 4 * nt), nt};
 
     // Inbound: all four arrays are read coalesced (element tid + m * nt)
     // into registers first, then go through LDS one after the other, so
@@ -981,20 +1108,26 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
     // compiler wait for each array's loads alone.
     double own[4][R], prv[4];
     double ld4[4][R];
+    // segments of the positions the thread assembles and extracts (row
+    // lines: p = tid + m * nt + 2)
+    int kk[K], ff[K], ll[K];
     if constexpr (ASSEMBLE && !INTERLEAVED) {
         // FS rows along i assembled here (as k_fs_assemble<0, 1>): position
         // p = tid + m * nt + 2 gives row p - 2, kept as ld4[.][m]
         const DevState &s = S[blockIdx.y];
         const double t = from_clock ? s.clock[0] : t_in;
+        row_segments<K>(s, line, tid, nt, kk, ff, ll);
+        bool r1[K], r2[K];  // the segments' wall tests, their loads in one round too
+#pragma unroll
+        for (int m = 0; m < K; m++) wall_flags(s, 0, line, ff[m], ll[m], r1[m], r2[m]);
 #pragma unroll
         for (int m = 0; m < K; m++) {
             const int p = tid + m * nt + 2;
             Row r = identity_row();
-            int k = -1, f = 0, l = 0;
+            const int k = kk[m], f = ff[m], l = ll[m];
             if (p <= s.M) {
-                if (locate<0>(s, line, p, k, f, l) && l - f >= 2) {
-                    bool ran1, ran2;
-                    wall_flags(s, 0, line, f, l, ran1, ran2);
+                if (k >= 0 && l - f >= 2) {
+                    const bool ran1 = r1[m], ran2 = r2[m];
                     if (p >= f + 1 && p <= l - 1) {
                         fs_row_i_fast(s, line, p, f, l, ran1, ran2, t, s_start, r.a, r.b, r.c, r.d);
                         if (p == f + 1) r.a = 0.0;
@@ -1018,12 +1151,13 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         // identity rows
         const DevState &s = S[blockIdx.y];
         const double f4 = 2 * s.c.g * s.c.dTchia2dY;
+        row_segments<K>(s, line, tid, nt, kk, ff, ll);
 #pragma unroll
         for (int m = 0; m < K; m++) {
             const int p = tid + m * nt + 2;
             Row rz = identity_row(), ru = identity_row();
-            int k, f, l;
-            if (p <= s.M && locate<0>(s, line, p, k, f, l)) {
+            const int k = kk[m], f = ff[m], l = ll[m];
+            if (p <= s.M && k >= 0) {
                 SegCtx sc;
                 hyd_rows<0, true>(s, line, p, f, l, f4, rz, ru, sc);
                 if (p == f) {
@@ -1124,17 +1258,13 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         const bool nb = tid + 1 < nt;
         const T yn = nb ? ey[tid + 1] : T(0), vn = nb ? ev[tid + 1] : T(0), wn = nb ? ew[tid + 1] : T(0);
         const T aK = la[K - 1], cK = lc[K - 1];
-        sh.a[tid] = aK * vv[K - 2];
-        sh.b[tid] = (T)red[K - 1].b + aK * wv[K - 2] + cK * vn;
-        sh.c[tid] = cK * wn;
-        sh.d[tid] = (T)red[K - 1].d - aK * yv[K - 2] - cK * yn;
+        pcr_solve<T>(sh, aK * vv[K - 2], (T)red[K - 1].b + aK * wv[K - 2] + cK * vn, cK * wn,
+                     (T)red[K - 1].d - aK * yv[K - 2] - cK * yn, REFINE);
     }
-    __syncthreads();
-    cr_solve<T>(sh, nt, REFINE);
 
     double zf[K];
     {
-        const T Lt = sh.d[tid], Lm = tid > 0 ? sh.d[tid - 1] : T(0);
+        const T Lt = sh.x()[tid], Lm = tid > 0 ? sh.x()[tid - 1] : T(0);
 #pragma unroll
         for (int k = 0; k < K - 1; k++) zf[k] = (double)(yv[k] + vv[k] * Lm + wv[k] * Lt);
         zf[K - 1] = (double)Lt;
@@ -1144,7 +1274,7 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         ez[tid] = zf[0];
         __syncthreads();
         const double zn = tid + 1 < nt ? ez[tid + 1] : 0.0;
-        const double zm = tid > 0 ? (double)sh.d[tid - 1] : 0.0;
+        const double zm = tid > 0 ? (double)sh.x()[tid - 1] : 0.0;
         T rv[K];
 #pragma unroll
         for (int k = 0; k < K; k++) {
@@ -1157,8 +1287,8 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
             rv[k] = (T)v;
         }
         // ... solved with the same elimination (v_k, w_k do not change).
-        // ey and sh.d are free again: every read of them happened before
-        // one of the barriers above.
+        // ey is free again: every read of it happened before one of the
+        // barriers above.
         T yr[K];
         yr[0] = rv[0] / den[0];
 #pragma unroll
@@ -1168,10 +1298,8 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         ey[tid] = yr[0];
         __syncthreads();
         const T yn = tid + 1 < nt ? ey[tid + 1] : T(0);
-        sh.d[tid] = rv[K - 1] - la[K - 1] * yr[K - 2] - lc[K - 1] * yn;
-        __syncthreads();
-        cr_replay<T>(sh, nt);
-        const T Ct = sh.d[tid], Cm = tid > 0 ? sh.d[tid - 1] : T(0);
+        pcr_replay<T>(sh, rv[K - 1] - la[K - 1] * yr[K - 2] - lc[K - 1] * yn);
+        const T Ct = sh.x()[tid], Cm = tid > 0 ? sh.x()[tid - 1] : T(0);
 #pragma unroll
         for (int k = 0; k < K - 1; k++) zf[k] = zf[k] + (double)(yr[k] + vv[k] * Cm + wv[k] * Ct);
         zf[K - 1] = zf[K - 1] + (double)Ct;
@@ -1210,12 +1338,19 @@ __global__ void __launch_bounds__(512) k_part(SweepBufs Bin, int start, int end,
         // along j coalesce) for every cell of the row, from the solution
         // staged in LDS: the solution never goes to HBM, and no extraction
         // launch follows
+        // (the positions are those of the assembly; their segments are
+        // looked up again, which costs less than holding them through the
+        // solve, where registers bound the workgroups per CU)
         const DevState &s = S[blockIdx.y];
-        for (int p = 2 + tid; p <= s.M; p += nt) {
+        row_segments<K>(s, line, tid, nt, kk, ff, ll);
+#pragma unroll
+        for (int m = 0; m < K; m++) {
+            const int p = 2 + tid + m * nt;
+            if (p > s.M || kk[m] < 0) continue;
             if constexpr (INTERLEAVED)
-                hyd_extract_cell<0>(s, line, p, [&](int rho) { return stage[rho + rho / R]; });
+                hyd_extract_at<0>(s, line, p, ff[m], ll[m], [&](int rho) { return stage[rho + rho / R]; });
             else
-                fs_extract_cell<0>(s, line, p, [&]() { return stage[(p - 2) + (p - 2) / R]; });
+                fs_extract_at<0>(s, line, p, ff[m], ll[m], [&]() { return stage[(p - 2) + (p - 2) / R]; });
         }
         return;
     }
@@ -1364,7 +1499,8 @@ static int allow_lds(hd_ctx *ctx, K kernel, size_t bytes) {
 template <typename T, bool REFINE>
 static size_t part_lds_bytes(int nt, int rows_per_thread) {
     const size_t stage = (size_t)nt * (rows_per_thread + 1) * sizeof(double);
-    const size_t other = (size_t)nt * 4 * sizeof(double) + (size_t)nt * (REFINE ? 6 : 4) * sizeof(T);
+    const size_t other = (size_t)nt * 4 * sizeof(double) +
+                         (size_t)nt * (10 + (REFINE ? 2 * pcr_steps(nt) : 0)) * sizeof(T);
     return stage > other ? stage : other;
 }
 
