@@ -118,6 +118,10 @@ def stage_bytes(name, units, sc, all_positions=False):
         "k_smooth": 8 + 4 + 8,        # t_vel + khouot in, tmp out
         "k_solve_v": 10 * 8 + 4 + 16,  # stencil fields in, t_v + t_u out
         "k_solve_u": 10 * 8 + 4 + 16,
+        # with sediment the post kernels also form VTH, Kx, Ky (three
+        # fields out; H_moi, and for k_post_b htaiz, in)
+        "k_post_a+vth": 6 * 8 + 4 + 6 * 8 + 8 + 3 * 8,
+        "k_post_b+vth": 5 * 8 + 5 * 8 + 2 * 8 + 3 * 8,
         "k_post_a": 6 * 8 + 4 + 6 * 8,
         "k_post_b": 5 * 8 + 5 * 8,
         "k_pre_reset": 3 * 8 + 4 + 16,
@@ -165,6 +169,8 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
             return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}>"
     if stage.startswith("k_thomas"):
         return "k_thomas"
+    if stage.endswith("+vth"):  # k_post_a / k_post_b with the VTH, K pass folded in
+        return stage[:-len("+vth")]
     # assembly / extraction: template <direction (1 = UZ / FS along j), layout
     # (0 = [rho][line] for the plain form, 1 = [line][rho])>
     # (column lines into [line][rho] are assembled by the LDS-tiled kernels)

@@ -106,6 +106,8 @@ static void derive_constants(DevState &d) {
     d.r_dX2 = 1.0 / d.c.dX2;
     d.r_dY2 = 1.0 / d.c.dY2;
     d.Dxr_m03 = std::pow(d.c.Dxr, -0.3);
+    d.qb_sg = std::pow((d.c.Sx - 1) * d.c.g, 0.5);
+    d.qb_dm = std::pow(d.c.dm, 1.5);
 }
 
 int sync_desc(hd_ctx *ctx) {
@@ -234,12 +236,14 @@ int run_halfstep_uz(hd_ctx *ctx, const DevState *S, const DevState &s, int n, do
     LAUNCH(ctx, "k_solve_v", k_solve_v<<<g.grid, g.block, 0, st>>>(S));
     LAUNCH(ctx, "k_margin<v>", k_margin<0><<<dim3((s.N - 1 + 63) / 64, n), 64, 0, st>>>(S));
     LAUNCH(ctx, "k_smooth<v>", k_smooth<false><<<g.grid, g.block, 0, st>>>(S));
-    LAUNCH(ctx, "k_post_a", k_post_a<<<g.grid, g.block, 0, st>>>(S));
+    // (with sediment, k_post_a also forms VTH and the diffusion
+    // coefficients of the FS pass below)
+    const bool sed = ctx->cfg.sediment && t >= ops.sediment_start;
+    LAUNCH(ctx, sed ? "k_post_a+vth" : "k_post_a", k_post_a<<<g.grid, g.block, 0, st>>>(S, sed ? 1 : 0));
     // khouot is unchanged in this half-step, so the segment tables are too;
     // only the FS boundary interpolation of Find_Calculation_limits runs.
     LAUNCH(ctx, "k_interp_fs", k_interp_fs<<<n, 1024, 0, st>>>(S));
-    if (ctx->cfg.sediment && t >= ops.sediment_start) {
-        LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
+    if (sed) {
         rc = run_line_sweep(ctx, S, s, n, 1, false, 3, s.M - 1, t, ops.bed_change_start);
         if (rc) return rc;
         LAUNCH(ctx, "k_update_fs", k_update_fs<<<g.grid, g.block, 0, st>>>(S));
@@ -274,11 +278,11 @@ int run_halfstep_vz_b(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     hipStream_t st = ctx->stream;
     const hd_options &ops = ctx->ops;
     Launch2D g = grid2d(s, n);
-    LAUNCH(ctx, "k_post_b", k_post_b<<<g.grid, g.block, 0, st>>>(S));
+    const bool sed = ctx->cfg.sediment && t >= ops.sediment_start;
+    LAUNCH(ctx, sed ? "k_post_b+vth" : "k_post_b", k_post_b<<<g.grid, g.block, 0, st>>>(S, sed ? 1 : 0));
     int rc = run_segmentation(ctx, S, s, n);
     if (rc) return rc;
-    if (ctx->cfg.sediment && t >= ops.sediment_start) {
-        LAUNCH(ctx, "k_vth_k", k_vth_k<<<g.grid, g.block, 0, st>>>(S));
+    if (sed) {
         rc = run_line_sweep(ctx, S, s, n, 0, false, 3, s.N - 1, t, ops.bed_change_start);
         if (rc) return rc;
         if (ctx->cfg.bed_change && t >= ops.bed_change_start) {

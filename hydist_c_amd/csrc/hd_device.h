@@ -31,7 +31,10 @@ struct DevState {
     // for the fast forms' assembly, which multiplies where the reference
     // divides; set from c with every descriptor upload
     double r_dXbp, r_dYbp, r_dX2, r_dY2;
-    double Dxr_m03;  // pow(c.Dxr, -0.3) of the non-cohesive source
+    double Dxr_m03;  // pow(c.Dxr, -0.3) of the non-cohesive source and of Qb
+    // the other constant powers of Calculate_Qb: pow((Sx - 1) g, 0.5) and
+    // pow(dm, 1.5) (the same values as the reference forms per cell)
+    double qb_sg, qb_dm;
     // inputs
     const double *h, *hsnham, *visc, *Fw, *Tsxw, *Tsyw;
     const double *bc[4], *CC[4];

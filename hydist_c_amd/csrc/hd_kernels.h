@@ -20,11 +20,11 @@ template <bool ISU> __global__ void k_smooth(const DevState *S);
 __global__ void k_solve_v(const DevState *S);
 __global__ void k_solve_u(const DevState *S);
 template <int DIR> __global__ void k_margin(const DevState *S);
-__global__ void k_post_a(const DevState *S);
+__global__ void k_post_a(const DevState *S, int vth);
 __global__ void k_pre_reset(const DevState *S);
 __global__ void k_rewet(const DevState *S);
 __global__ void k_dry(const DevState *S);
-__global__ void k_post_b(const DevState *S);
+__global__ void k_post_b(const DevState *S, int vth);
 __global__ void k_interp_fs(const DevState *S);
 __global__ void k_vth_k(const DevState *S);
 __global__ void k_update_fs(const DevState *S);

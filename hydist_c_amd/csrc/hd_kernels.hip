@@ -117,14 +117,18 @@ HD_DEV void segment_rows_block(const DevState &s, int b) {
 // Columns (lines along i, stride P): a workgroup takes SEG_COLS adjacent
 // columns (64 bytes of khouot per row) and splits the rows 2..N into
 // SEG_GROUPS contiguous groups, one thread per (group, column).  Pass 1
-// counts run starts per group, an LDS prefix gives each group its first
-// segment index, pass 2 writes the run starts and ends.  Both passes load
+// counts run starts per group and finds its first dry row; LDS then gives
+// each group its first segment index, the start of a run carried in from
+// the groups before and the first dry row after it.  Pass 2 writes the
+// run starts and ends, and per cell the segment index and first row;
+// pass 3, backward, per cell the segment's last row.  Every pass loads
 // SEG_TILE rows at once (clamped to the column), so one memory latency
 // covers a batch of rows.
 // Needs blockDim.x == SEG_COLS * SEG_GROUPS (every thread reaches the barrier).
 HD_DEV void segment_cols_block(const DevState &s, int b) {
     __shared__ int cnt[SEG_GROUPS][SEG_COLS];
     __shared__ int lastst[SEG_GROUPS][SEG_COLS];
+    __shared__ int firstdry[SEG_GROUPS][SEG_COLS];
     const int tx = threadIdx.x % SEG_COLS, g = threadIdx.x / SEG_COLS;
     const int j = b * SEG_COLS + tx + 2;
     const int end = s.N, W = s.P;
@@ -133,7 +137,7 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
     const int r0 = 2 + g * per, r1 = min(end + 1, r0 + per);  // [r0, r1)
     const bool live = j <= s.M;
     const int *__restrict__ kh = s.khouot;
-    int n = 0, ls = -1;
+    int n = 0, ls = -1, fd = end + 1;
     if (live && r0 < r1) {
         bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
         for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
@@ -149,6 +153,7 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
                         n++;
                         ls = p;
                     }
+                    if (!wet && fd > end) fd = p;
                     prev = wet;
                 }
             }
@@ -156,19 +161,32 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
     }
     cnt[g][tx] = n;
     lastst[g][tx] = ls;
+    firstdry[g][tx] = fd;
     __syncthreads();
-    int off = 0;
-    for (int q = 0; q < g; q++) off += cnt[q][tx];
+    int off = 0, cs = -1;  // segments before the group; the last run start before it
+    for (int q = 0; q < g; q++) {
+        off += cnt[q][tx];
+        cs = max(cs, lastst[q][tx]);
+    }
+    int ce = end + 1;  // the first dry row after the group (end + 1: none)
+    for (int q = g + 1; q < SEG_GROUPS; q++) ce = min(ce, firstdry[q][tx]);
     // the column's last run start: a run starting at end-1 or later is no
     // segment, and it is always the last run
     int lst_all = -1;
     for (int q = 0; q < SEG_GROUPS; q++) lst_all = max(lst_all, lastst[q][tx]);
     const int dropped_from = lst_all >= end - 1 ? lst_all : end + 1;
+    // Pass 2, forward: the tables, each cell's segment index and its
+    // segment's first row (the run start, carried in from the groups
+    // before when the run began there).  Pass 3, backward: each cell's
+    // segment's last row (the row before the next dry one, carried in from
+    // the groups after).  Cells outside segments get -1.
+    int *fl = s.segflV;
     if (live && r0 < r1) {
         int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
         int *segid = s.segidV + j;
         int k = off;  // starts at rows <= p
         bool prev = r0 > 2 && kh[(r0 - 1) * W + j] == 0;
+        int start = cs;  // start of the run through p
         for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
             int v[SEG_TILE + 1];  // rows p0 .. p0 + SEG_TILE (the last for the end test)
 #pragma unroll
@@ -179,36 +197,32 @@ HD_DEV void segment_cols_block(const DevState &s, int b) {
                 if (p < r1) {
                     const bool wet = v[u] == 0;
                     const bool nwet = p < end && v[u + 1] == 0;
-                    if (wet && !prev) first[k++] = p;
+                    if (wet && !prev) {
+                        first[k++] = p;
+                        start = p;
+                    }
                     if (wet && !nwet) last[k - 1] = p;
-                    segid[(size_t)p * W] = (wet && p < dropped_from) ? k - 1 : -1;
+                    const bool in = wet && p < dropped_from;
+                    segid[(size_t)p * W] = in ? k - 1 : -1;
+                    fl[2 * ((size_t)p * W + j)] = in ? start : -1;
                     prev = wet;
                 }
             }
         }
-    }
-    // pass 3, once every group's starts and ends are in the tables: each
-    // cell's segment bounds beside it
-    __syncthreads();
-    // (SEG_TILE rows at a time, their loads unconditional and clamped to the
-    // group, so the index and table loads of a batch overlap)
-    if (live && r0 < r1) {
-        const int *first = s.segV_f + (size_t)j * s.capV, *last = s.segV_l + (size_t)j * s.capV;
-        for (int p0 = r0; p0 < r1; p0 += SEG_TILE) {
-            int kk[SEG_TILE];
+        int nd = ce;  // the first dry row after p
+        for (int p0 = r0 + (r1 - 1 - r0) / SEG_TILE * SEG_TILE; p0 >= r0; p0 -= SEG_TILE) {
+            int v[SEG_TILE];
 #pragma unroll
-            for (int u = 0; u < SEG_TILE; u++) kk[u] = s.segidV[(size_t)min(p0 + u, r1 - 1) * W + j];
-            int2 fl[SEG_TILE];
+            for (int u = 0; u < SEG_TILE; u++) v[u] = kh[min(p0 + u, r1 - 1) * W + j];
 #pragma unroll
-            for (int u = 0; u < SEG_TILE; u++) {
-                const int k = max(kk[u], 0);
-                fl[u] = make_int2(first[k], last[k]);
+            for (int u = SEG_TILE - 1; u >= 0; u--) {
+                const int p = p0 + u;
+                if (p < r1) {
+                    const bool wet = v[u] == 0;
+                    if (!wet) nd = p;
+                    fl[2 * ((size_t)p * W + j) + 1] = (wet && p < dropped_from) ? nd - 1 : -1;
+                }
             }
-#pragma unroll
-            for (int u = 0; u < SEG_TILE; u++)
-                if (p0 + u < r1)
-                    *reinterpret_cast<int2 *>(s.segflV + 2 * ((size_t)(p0 + u) * W + j)) =
-                        kk[u] >= 0 ? fl[u] : make_int2(-1, -1);
         }
     }
     if (g == 0 && live) {
@@ -536,27 +550,43 @@ HD_DEV double z_after(const DevState &s, int i, int j) {
     return (i <= s.N && j <= s.M) ? s.t_z[pos] : s.z[pos];
 }
 
+HD_DEV void vth_k_cell(const DevState &s, int i, int j, int pos, double htz, double H, double tu, double tu_w,
+                       double tv, double tv_1, double htdu, double htdv, double h0, double h_1, double h_w);
+
 // End of the first half-step (engine.cpp:122-139): update_buffer(v),
-// update_h_moi, update_uvz and Htuongdoi in one pass.
-__global__ void k_post_a(const DevState *__restrict__ S) {
+// update_h_moi, update_uvz and Htuongdoi in one pass; with vth, also the
+// sediment pass's Find_VTH and hesoK (sediment_transport.cpp:539-602),
+// which read only values this pass has at hand (the face velocities
+// t_v at pos - 1 as update_buffer leaves them).
+__global__ void k_post_a(const DevState *__restrict__ S, int vth) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     if (i > s.N + 2 || j > s.M + 2) return;
     const int W = s.P, pos = i * W + j;
     if (interior(s, i, j)) s.t_v[pos] = s.tmp[pos];
-    if (i >= 1 && j >= 1 && i <= s.N && j <= s.M && s.khouot[pos] == 0)
-        s.H_moi[pos] = s.htaiz[pos] + s.t_z[pos];
+    const bool cell = i >= 1 && j >= 1 && i <= s.N && j <= s.M;
+    const bool wet = cell && s.khouot[pos] == 0;
+    if (wet) s.H_moi[pos] = s.htaiz[pos] + s.t_z[pos];
     const int kd = (int)s.c.kenhhepd, kn = (int)s.c.kenhhepng;
     if (i <= s.N && j <= s.M) {
         s.u[pos] = s.t_u[pos] * (1 - kd);
         s.v[pos] = (interior(s, i, j) ? s.tmp[pos] : s.t_v[pos]) * (1 - kn);
     }
-    if (i >= 1 && j >= 1 && i <= s.N && j <= s.M) {
+    if (cell) {
         const double *h = s.h;
         const double zij = s.t_z[pos];
-        s.Htdu[pos] = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
-        s.Htdv[pos] = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+        const double htdu = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
+        const double htdv = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+        s.Htdu[pos] = htdu;
+        s.Htdv[pos] = htdv;
+        if (vth) {
+            const double H = wet ? s.htaiz[pos] + zij : s.H_moi[pos];
+            const double tv = interior(s, i, j) ? s.tmp[pos] : s.t_v[pos];
+            const double tv_1 = interior(s, i, j - 1) ? s.tmp[pos - 1] : s.t_v[pos - 1];
+            vth_k_cell(s, i, j, pos, s.htaiz[pos], H, s.t_u[pos], s.t_u[pos - W], tv, tv_1, htdu, htdv, h[pos],
+                       h[pos - 1], h[pos - W]);
+        }
     }
     // z last: other threads read z only outside [0,N]x[0,M], which is not written
     if (i <= s.N && j <= s.M) s.z[pos] = s.t_z[pos];
@@ -653,21 +683,29 @@ __global__ void k_dry(const DevState *__restrict__ S) {
     }
 }
 
-// End of the second half-step (engine.cpp:247-255): update_uvz and Htuongdoi.
-__global__ void k_post_b(const DevState *__restrict__ S) {
+// End of the second half-step (engine.cpp:247-255): update_uvz and
+// Htuongdoi; with vth, also Find_VTH and hesoK of the sediment pass that
+// follows (the segmentation between them changes nothing they read).
+__global__ void k_post_b(const DevState *__restrict__ S, int vth) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     if (i > s.N || j > s.M) return;
     const int W = s.P, pos = i * W + j;
     const int kd = (int)s.c.kenhhepd, kn = (int)s.c.kenhhepng;
-    s.u[pos] = s.t_u[pos] * (1 - kd);
-    s.v[pos] = s.t_v[pos] * (1 - kn);
+    const double tu = s.t_u[pos], tv = s.t_v[pos];
+    s.u[pos] = tu * (1 - kd);
+    s.v[pos] = tv * (1 - kn);
     if (i >= 1 && j >= 1) {
         const double *h = s.h;
         const double zij = s.t_z[pos];
-        s.Htdu[pos] = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
-        s.Htdv[pos] = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+        const double htdu = (h[i * W + j - 1] + h[pos] + z_after(s, i + 1, j) + zij) * 0.5;
+        const double htdv = (h[(i - 1) * W + j] + h[pos] + z_after(s, i, j + 1) + zij) * 0.5;
+        s.Htdu[pos] = htdu;
+        s.Htdv[pos] = htdv;
+        if (vth)
+            vth_k_cell(s, i, j, pos, s.htaiz[pos], s.H_moi[pos], tu, s.t_u[pos - W], tv, s.t_v[pos - 1], htdu, htdv,
+                       h[pos], h[pos - 1], h[pos - W]);
     }
     s.z[pos] = s.t_z[pos];
 }
@@ -734,32 +772,43 @@ __global__ void __launch_bounds__(SEG_COLS *SEG_GROUPS) k_segment(const DevState
 // Sediment
 // ---------------------------------------------------------------------------
 
-// Find_VTH (sediment_transport.cpp:575-602) and hesoK (:539-573).
+// Find_VTH (sediment_transport.cpp:575-602) and hesoK (:539-573) of cell
+// (i, j) in [1, N] x [1, M], from the values they read: htaiz, H_moi, t_u
+// at pos and pos - W, t_v at pos and pos - 1, Htdu, Htdv, and h at pos,
+// pos - 1 and pos - W.
+HD_DEV void vth_k_cell(const DevState &s, int i, int j, int pos, double htz, double H, double tu, double tu_w,
+                       double tv, double tv_1, double htdu, double htdv, double h0, double h_1, double h_w) {
+    if (i >= 2 && j >= 2) {
+        if ((htz > s.c.NANGDAY) && (H > s.c.H_TINH)) {
+            double ut = (tu + tu_w) * 0.5;
+            double vt = (tv + tv_1) * 0.5;
+            s.VTH[pos] = sqrt(ut * ut + vt * vt);
+        } else
+            s.VTH[pos] = 0;
+    }
+    const double Ks = s.c.Ks, g = s.c.g;
+    if (htdu > 0) {
+        double Cz = 18 * log(12 * htdu / Ks);
+        double k = 5.93 * sqrt(g) * (h_1 + h0) * 0.25 * fabs(tu) / Cz;
+        s.Kx[pos] = fmin(100.0, fmax(5.0, k));
+    }
+    if (htdv > 0) {
+        double Cz = 18 * log(12 * htdv / Ks);
+        double k = 5.93 * sqrt(g) * (h_w + h0) * 0.25 * fabs(tv) / Cz;
+        s.Ky[pos] = fmin(100.0, fmax(5.0, k));
+    }
+}
+
+// The same as a pass of its own (the stage-by-stage entry points; a step
+// forms them inside k_post_a / k_post_b).
 __global__ void k_vth_k(const DevState *__restrict__ S) {
     const DevState &s = S[blockIdx.z];
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y * blockDim.y + threadIdx.y;
     if (i < 1 || j < 1 || i > s.N || j > s.M) return;
     const int W = s.P, pos = i * W + j;
-    if (i >= 2 && j >= 2) {
-        if ((s.htaiz[pos] > s.c.NANGDAY) && (s.H_moi[pos] > s.c.H_TINH)) {
-            double ut = (s.t_u[pos] + s.t_u[pos - W]) * 0.5;
-            double vt = (s.t_v[pos] + s.t_v[pos - 1]) * 0.5;
-            s.VTH[pos] = sqrt(ut * ut + vt * vt);
-        } else
-            s.VTH[pos] = 0;
-    }
-    const double Ks = s.c.Ks, g = s.c.g;
-    if (s.Htdu[pos] > 0) {
-        double Cz = 18 * log(12 * s.Htdu[pos] / Ks);
-        double k = 5.93 * sqrt(g) * (s.h[pos - 1] + s.h[pos]) * 0.25 * fabs(s.t_u[pos]) / Cz;
-        s.Kx[pos] = fmin(100.0, fmax(5.0, k));
-    }
-    if (s.Htdv[pos] > 0) {
-        double Cz = 18 * log(12 * s.Htdv[pos] / Ks);
-        double k = 5.93 * sqrt(g) * (s.h[pos - W] + s.h[pos]) * 0.25 * fabs(s.t_v[pos]) / Cz;
-        s.Ky[pos] = fmin(100.0, fmax(5.0, k));
-    }
+    vth_k_cell(s, i, j, pos, s.htaiz[pos], s.H_moi[pos], s.t_u[pos], s.t_u[pos - W], s.t_v[pos], s.t_v[pos - 1],
+               s.Htdu[pos], s.Htdv[pos], s.h[pos], s.h[pos - 1], s.h[pos - W]);
 }
 
 
@@ -789,10 +838,11 @@ __global__ void k_qb(const DevState *__restrict__ S) {
         double Tob = c.ro * s.Fw[pos] * (vth * vth);
         if (s.H_moi[pos] > c.ghtoe) Toee = c.Toe * (1 + c.hstoe * ((s.H_moi[pos] - c.ghtoe)));
         if (Tob > Toee) Tx = Tob / Toee - 1;
-        qx = 0.053 * pow((c.Sx - 1) * c.g, 0.5) * pow(c.dm, 1.5) * pow(Tx, 2.1) * pow(c.Dxr, -0.3) *
-             (s.t_u[pos] + s.t_u[pos - W]) * 0.5 / vth;
-        qy = 0.053 * pow((c.Sx - 1) * c.g, 0.5) * pow(c.dm, 1.5) * pow(Tx, 2.1) * pow(c.Dxr, -0.3) *
-             (s.t_v[pos] + s.t_v[pos - 1]) * 0.5 / vth;
+        // the constant powers come with the descriptor, computed once on
+        // the host; the products run in the reference's order
+        const double ptx = pow(Tx, 2.1);
+        qx = 0.053 * s.qb_sg * s.qb_dm * ptx * s.Dxr_m03 * (s.t_u[pos] + s.t_u[pos - W]) * 0.5 / vth;
+        qy = 0.053 * s.qb_sg * s.qb_dm * ptx * s.Dxr_m03 * (s.t_v[pos] + s.t_v[pos - 1]) * 0.5 / vth;
     }
     const int N = s.N, M = s.M;
     double *Qbx = s.Qbx, *Qby = s.Qby;
