@@ -1,0 +1,46 @@
+"""Small cases of the GPU tests, run on the CPU: the library's kernels
+built against the emulated HIP runtime (tests/emu: every lane on the host,
+device buffers and dynamic LDS against guard pages) and checked by the
+same test code as on the GPU (HD_EMU=1), in a child process so the
+emulated library never meets the GPU build in one interpreter.  This puts
+the kernels' indexing and arithmetic (bitwise plain form, fast forms at
+their bars, every line solver, the tridiagonal entry points, degenerate
+meshes) under the CPU suite; timing and occupancy stay GPU questions."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+FILES = ["test_gpu_gtsv.py", "test_zz_no_alloc.py", "test_gpu_parity.py", "test_gpu_fast.py",
+         "test_gpu_edge_meshes.py"]
+# the cases that finish in about a minute on the emulator together
+SELECT = " or ".join([
+    "gtsv", "noalloc",
+    "(test_plain_fp64_bitwise and (12_0 or 12_1 or -20] or -25]))",
+    "half_steps", "sediment_and_bed_change", "lake_at_rest", "stage_entry_points", "q11_literal",
+    "init_matches", "mass_balance_gpu", "output_callback",
+    "(test_fast_form_close_to_oracle and 10_)", "test_fast_form_sediment",
+    "(test_every_line_solver_close_to_oracle and part4)",
+    "edge_mesh_plain", "(edge_mesh_fast and (stripes or onecell or alland))",
+])
+
+
+@pytest.fixture(scope="module")
+def emu_built():
+    sys.path.insert(0, os.path.join(HERE, "emu"))
+    import build_emu
+    build_emu.build()
+
+
+def test_gpu_suite_small_cases_on_the_emulator(emu_built):
+    env = dict(os.environ, HD_EMU="1")
+    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-k", SELECT] + \
+          [os.path.join(HERE, f) for f in FILES]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, cwd=os.path.dirname(HERE), timeout=900)
+    tail = r.stdout[-3000:] + r.stderr[-2000:]
+    assert r.returncode == 0, tail
+    assert " passed" in r.stdout and "deselected" in r.stdout, tail
+    print(r.stdout.strip().splitlines()[-1])
