@@ -9,7 +9,9 @@
 // runs' kernels overlap on the device; then it retires finished runs and
 // admits queued ones while the HBM budget and the slot limit allow.  A run
 // joins or leaves at a round boundary, so a late arrival does not wait for
-// the runs already on the GPU to finish.
+// the runs already on the GPU to finish.  Admitted runs get their context
+// (allocation, upload, init) on a helper thread, so a large run's creation
+// does not hold up the quanta of the runs already on the GPU.
 //
 // A run's arithmetic never depends on what else is running: its result is
 // the same as when it runs alone.
@@ -37,6 +39,7 @@ using Clock = std::chrono::steady_clock;
 
 constexpr int kResultFields[] = {HD_F_U, HD_F_V, HD_F_Z, HD_F_FS, HD_F_DH};
 constexpr int kNumResults = 6;  // the five above + khouot
+constexpr int kQuantumSpread = 4;  // most steps per round of a small run, in quanta
 
 struct Run {
     int64_t id = 0;
@@ -79,12 +82,15 @@ struct hd_server {
     std::mutex mu;
     std::condition_variable wake, done;
     std::deque<Run *> queue;
+    std::deque<Run *> to_create;  // admitted, waiting for the helper thread
+    int creating = 0;             // admitted runs without a context yet
     std::vector<Run *> active;
     std::map<int64_t, std::unique_ptr<Run>> runs;
     size_t used = 0;
     int64_t next_id = 1;
     bool stop = false;
-    std::thread worker;
+    std::thread worker, creator;
+    std::condition_variable create_cv;
     // statistics
     std::vector<double> latency;  // seconds, completed runs
     double cell_steps = 0;
@@ -94,8 +100,19 @@ struct hd_server {
     bool any = false;
 
     void loop();
+    void create_loop();
+    bool admissible() const;
     void finish(Run *r, int state);
 };
+
+// Whether the worker can act on the head of the queue now: a free slot and
+// a head that fits the budget, or one that leaves the queue anyway.  Called
+// with mu held.
+bool hd_server::admissible() const {
+    if (queue.empty() || (int)active.size() + creating >= cfg.max_active) return false;
+    const Run *r = queue.front();
+    return r->cancel || r->footprint > cfg.hbm_budget || used + r->footprint <= cfg.hbm_budget;
+}
 
 void hd_server::finish(Run *r, int state) {
     // called by the worker without the lock held
@@ -136,18 +153,56 @@ void hd_server::finish(Run *r, int state) {
         failed++;
     }
     done.notify_all();
+    wake.notify_all();  // budget freed: the queue's head may fit now
+}
+
+// The helper thread: creates and initialises the context of each admitted
+// run in admission order, then hands the run to the worker.
+void hd_server::create_loop() {
+    (void)hipSetDevice(cfg.device);
+    for (;;) {
+        Run *r;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            create_cv.wait(lk, [&] { return stop || !to_create.empty(); });
+            if (stop) return;
+            r = to_create.front();
+            to_create.pop_front();
+        }
+        r->started = Clock::now();
+        int rc = hd_create(&r->mesh, &r->coeffs, &r->ops, &r->cfg, &r->ctx);
+        if (!rc) rc = hd_set_graph(r->ctx, cfg.use_graph);
+        if (!rc) rc = hd_init(r->ctx);
+        if (rc) {
+            r->rc = rc;
+            r->err = hd_last_error(r->ctx);
+        }
+        bool cancel;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            creating--;
+            cancel = r->cancel;
+            if (!rc && !cancel) {
+                r->state = HD_RUN_RUNNING;
+                active.push_back(r);
+            }
+            wake.notify_all();
+        }
+        if (rc) finish(r, HD_RUN_FAILED);
+        else if (cancel) finish(r, HD_RUN_CANCELLED);
+    }
 }
 
 void hd_server::loop() {
     (void)hipSetDevice(cfg.device);
     for (;;) {
-        std::vector<Run *> admit, dropped;
         {
             std::unique_lock<std::mutex> lk(mu);
-            wake.wait(lk, [&] { return stop || !queue.empty() || !active.empty(); });
+            wake.wait(lk, [&] { return stop || !active.empty() || admissible(); });
             if (stop) return;
-            // FIFO admission by HBM footprint and slot count
-            while (!queue.empty() && (int)(active.size() + admit.size()) < cfg.max_active) {
+            // FIFO admission by HBM footprint and slot count; the contexts
+            // are made on the helper thread
+            while (!queue.empty() && (int)active.size() + creating < cfg.max_active) {
                 Run *r = queue.front();
                 if (r->cancel) {
                     queue.pop_front();
@@ -169,33 +224,28 @@ void hd_server::loop() {
                 queue.pop_front();
                 used += r->footprint;
                 admitted++;
-                admit.push_back(r);
+                creating++;
+                to_create.push_back(r);
+                create_cv.notify_one();
             }
         }
-        for (Run *r : admit) {
-            r->started = Clock::now();
-            int rc = hd_create(&r->mesh, &r->coeffs, &r->ops, &r->cfg, &r->ctx);
-            if (!rc) rc = hd_set_graph(r->ctx, cfg.use_graph);
-            if (!rc) rc = hd_init(r->ctx);
-            if (rc) {
-                r->rc = rc;
-                r->err = hd_last_error(r->ctx);
-                dropped.push_back(r);
-                continue;
-            }
-            std::lock_guard<std::mutex> lk(mu);
-            r->state = HD_RUN_RUNNING;
-            active.push_back(r);
-        }
-        for (Run *r : dropped) finish(r, HD_RUN_FAILED);
         std::vector<Run *> cur;
         {
             std::lock_guard<std::mutex> lk(mu);
             cur = active;
         }
-        // one quantum per run, each on its own stream
+        // One quantum per run, each on its own stream.  The largest run
+        // takes steps_per_quantum steps and sets the round's length; a
+        // smaller run takes more steps in the same round, in inverse
+        // proportion to its cell count (up to kQuantumSpread times as
+        // many), so small runs finish and free their slots sooner instead
+        // of advancing in lockstep with the largest one.
+        double cells_max = 0;
+        for (Run *r : cur) cells_max = std::max(cells_max, (double)r->mesh.M * r->mesh.N);
         for (Run *r : cur) {
-            for (int k = 0; k < cfg.steps_per_quantum && r->t < r->T && !r->rc; k++) {
+            const double ratio = cells_max / ((double)r->mesh.M * r->mesh.N);
+            const int quantum = cfg.steps_per_quantum * std::max(1, std::min(kQuantumSpread, (int)ratio));
+            for (int k = 0; k < quantum && r->t < r->T && !r->rc; k++) {
                 int rc = hd_step(r->ctx, &r->t);
                 if (rc) {
                     r->rc = rc;
@@ -250,6 +300,7 @@ int hd_server_create(const hd_server_config *cfg, hd_server **out) {
     hd_server *s = new hd_server();
     s->cfg = *cfg;
     s->worker = std::thread([s] { s->loop(); });
+    s->creator = std::thread([s] { s->create_loop(); });
     *out = s;
     return HD_OK;
 }
@@ -403,8 +454,12 @@ int hd_server_destroy(hd_server *s) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
         s->wake.notify_all();
+        s->create_cv.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    if (s->creator.joinable()) s->creator.join();
+    // runs the helper thread had made a context for but not yet handed over
+    // are in `active` too; runs still waiting for one have none
     for (Run *r : s->active)
         if (r->ctx) hd_destroy(r->ctx);
     delete s;

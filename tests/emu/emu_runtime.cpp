@@ -6,7 +6,9 @@
 // allocation the address belongs to.  A launch runs its workgroups in
 // order; a workgroup's lanes are ucontext fibers and __syncthreads is a
 // yield: a pass resumes every live lane once, so no lane leaves a barrier
-// before every lane has reached it.
+// before every lane has reached it.  Launches and allocations from several
+// host threads (the serving loop's worker and its context-making helper)
+// take one lock, so they run one after another, as on one queue.
 #include <signal.h>
 #include <sys/mman.h>
 #include <ucontext.h>
@@ -30,6 +32,7 @@ struct Alloc {
 };
 std::map<uintptr_t, Alloc> g_allocs;  // by user pointer
 size_t g_page = 4096;
+std::recursive_mutex g_mu;  // launches and the allocation map
 
 void *guarded(size_t n, Alloc *out) {
     const size_t body = (n + 15) / 16 * 16;
@@ -76,6 +79,7 @@ struct Node {
 std::vector<Node> *g_capture = nullptr;
 
 void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function<void()> &body) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     const unsigned nt = b.x * b.y * b.z;
     if (nt == 0 || nt > 1024) {
         std::fprintf(stderr, "[emu] %s: bad block size %u\n", name, nt);
@@ -214,6 +218,7 @@ hipError_t hipGetDeviceProperties(hipDeviceProp_t *p, int) {
     return hipSuccess;
 }
 hipError_t hipMalloc(void **p, size_t n) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     Alloc a{};
     void *u = guarded(n ? n : 1, &a);
     if (!u) return hipErrorOutOfMemory;
@@ -222,6 +227,7 @@ hipError_t hipMalloc(void **p, size_t n) {
     return hipSuccess;
 }
 hipError_t hipFree(void *p) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     auto it = g_allocs.find((uintptr_t)p);
     if (it == g_allocs.end()) return p ? hipErrorInvalidValue : hipSuccess;
     munmap(it->second.base, it->second.total);
@@ -232,6 +238,7 @@ hipError_t hipMallocAsync(void **p, size_t n, hipStream_t) { return hipMalloc(p,
 hipError_t hipFreeAsync(void *p, hipStream_t) { return hipFree(p); }
 hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
 void *emu_symbol_buffer(const void *sym, size_t n) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     static std::map<const void *, void *> bufs;
     auto it = bufs.find(sym);
     if (it != bufs.end()) return it->second;

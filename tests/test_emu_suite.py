@@ -5,7 +5,8 @@ same test code as on the GPU (HD_EMU=1), in a child process so the
 emulated library never meets the GPU build in one interpreter.  This puts
 the kernels' indexing and arithmetic (bitwise plain form, fast forms at
 their bars, every line solver, the tridiagonal entry points, degenerate
-meshes) under the CPU suite; timing and occupancy stay GPU questions."""
+meshes, the serving loop with its worker and context-making threads)
+under the CPU suite; timing and occupancy stay GPU questions."""
 import os
 import subprocess
 import sys
@@ -15,10 +16,10 @@ import pytest
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 FILES = ["test_gpu_gtsv.py", "test_zz_no_alloc.py", "test_gpu_parity.py", "test_gpu_fast.py",
-         "test_gpu_edge_meshes.py"]
-# the cases that finish in about a minute on the emulator together
+         "test_gpu_edge_meshes.py", "test_gpu_server.py"]
+# the cases that finish in about two minutes on the emulator together
 SELECT = " or ".join([
-    "gtsv", "noalloc",
+    "gtsv", "noalloc", "test_server",
     "(test_plain_fp64_bitwise and (12_0 or 12_1 or -20] or -25]))",
     "half_steps", "sediment_and_bed_change", "lake_at_rest", "stage_entry_points", "q11_literal",
     "init_matches", "mass_balance_gpu", "output_callback",
