@@ -376,16 +376,18 @@ int hd_memory_bytes(hd_ctx *ctx, size_t *bytes);
 /* process, main.cpp): what-if runs are submitted as they arrive; a    */
 /* worker thread admits them in order while their footprint fits the   */
 /* HBM budget and a slot is free, keeps each admitted run resident in  */
-/* its own context and stream, and advances all admitted runs together */
-/* in rounds of steps_per_quantum steps (their kernels overlap on the  */
-/* device).  A run's result does not depend on what else runs.         */
+/* its own context and stream (made on a helper thread), and advances  */
+/* all admitted runs together in rounds: the largest run takes         */
+/* steps_per_quantum steps per round, a smaller one up to 4x as many,  */
+/* in inverse proportion to its cell count (their kernels overlap on   */
+/* the device).  A run's result does not depend on what else runs.     */
 /* ------------------------------------------------------------------ */
 typedef struct hd_server hd_server;
 
 typedef struct hd_server_config {
     size_t hbm_budget;      /* device bytes the admitted runs may hold together */
     int max_active;         /* admitted runs at most */
-    int steps_per_quantum;  /* steps per run per round */
+    int steps_per_quantum;  /* steps per round of the largest running run */
     int use_graph;          /* 1: each step is one graph launch */
     int device;
 } hd_server_config;
