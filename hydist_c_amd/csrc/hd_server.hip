@@ -3,18 +3,26 @@
 // The reference runs one scenario per process, start to finish
 // (main.cpp:138-309).  The server instead takes runs as they arrive
 // (different meshes, hydrographs, roughness, durations, forms), keeps each
-// admitted run's state resident in HBM in its own context, and advances
-// all admitted runs together: every round issues a quantum of steps for
-// each run, each on its own stream (one graph launch per step), so the
-// runs' kernels overlap on the device; then it retires finished runs and
-// admits queued ones while the HBM budget and the slot limit allow.  A run
+// admitted run's state resident in HBM, and advances all admitted runs
+// together: every round issues a quantum of steps for each group of runs,
+// each group on its own stream (one graph launch per step), so the groups'
+// kernels overlap on the device; then it retires finished groups and
+// admits queued runs while the HBM budget and the slot limit allow.  A run
 // joins or leaves at a round boundary, so a late arrival does not wait for
-// the runs already on the GPU to finish.  Admitted runs get their context
-// (allocation, upload, init) on a helper thread, so a large run's creation
-// does not hold up the quanta of the runs already on the GPU.
+// the runs already on the GPU to finish.
+//
+// A group is one context.  Queued runs that can share launches (the same
+// mesh size, duration, form, options and coefficients; their inputs, such
+// as depths, roughness and boundary series, may differ) are admitted
+// together into one batched context (hd_batch_create), so every kernel
+// launch of a step serves all of them (the scenario is the grid's outer
+// dimension); other runs get a context of their own.  Contexts are made on
+// a helper thread, so a large run's allocation and upload do not hold up
+// the quanta of the groups already on the GPU.
 //
 // A run's arithmetic never depends on what else is running: its result is
-// the same as when it runs alone.
+// the same as when it runs alone (a batched context computes each scenario
+// as a context of its own would).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,9 +45,12 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int kResultFields[] = {HD_F_U, HD_F_V, HD_F_Z, HD_F_FS, HD_F_DH};
-constexpr int kNumResults = 6;  // the five above + khouot
-constexpr int kQuantumSpread = 4;  // most steps per round of a small run, in quanta
+constexpr int kResultFields[] = {HD_F_U, HD_F_V, HD_F_Z, HD_F_FS, HD_F_DH, HD_F_KHOUOT};
+constexpr int kNumResults = 6;
+constexpr int kQuantumSpread = 4;  // most steps per round of a small group, in quanta
+constexpr int kMaxBatch = 16;      // runs per batched context at most
+
+struct Group;
 
 struct Run {
     int64_t id = 0;
@@ -50,12 +61,13 @@ struct Run {
     hd_coeffs coeffs{};
     hd_options ops{};
     hd_config cfg{};
-    size_t footprint = 0;
+    size_t footprint = 0;  // of the run in a context of its own
     int state = HD_RUN_QUEUED;
     double t = 0.0;
     int T = 0;  // Tmax truncated as engine.cpp:62
     long steps = 0;
-    hd_ctx *ctx = nullptr;
+    Group *group = nullptr;
+    int scen = 0;  // the run's scenario in its group's context
     bool cancel = false;
     int rc = 0;
     std::string err;
@@ -63,10 +75,23 @@ struct Run {
     std::vector<double> results[kNumResults];
 };
 
+// Runs advanced by one context: scenario k is members[k].
+struct Group {
+    std::vector<Run *> members;
+    hd_ctx *ctx = nullptr;
+    double t = 0.0;
+    int T = 0;
+    long steps = 0;
+    size_t footprint = 0;
+    int rc = 0;
+    std::string err;
+    double cells() const { return (double)members[0]->mesh.M * members[0]->mesh.N * members.size(); }
+};
+
 int result_slot(int field) {
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < kNumResults; k++)
         if (kResultFields[k] == field) return k;
-    return field == HD_F_KHOUOT ? 5 : -1;
+    return -1;
 }
 
 const double *own(std::vector<double> &dst, const double *src, size_t n) {
@@ -75,26 +100,40 @@ const double *own(std::vector<double> &dst, const double *src, size_t n) {
     return dst.data();
 }
 
+// Whether two runs can share one batched context: one mesh size and series
+// length, one duration and start, one form and configuration, the same
+// coefficients and the same options (hd_batch_create takes one of each).
+bool same_launches(const Run &a, const Run &b) {
+    const hd_options &x = a.ops, &y = b.ops;
+    return a.mesh.M == b.mesh.M && a.mesh.N == b.mesh.N && a.mesh.total_time == b.mesh.total_time && a.T == b.T &&
+           a.t == b.t && std::memcmp(&a.cfg, &b.cfg, sizeof(hd_config)) == 0 &&
+           std::memcmp(&a.coeffs, &b.coeffs, sizeof(hd_coeffs)) == 0 && x.Tmax == y.Tmax &&
+           x.t_start == y.t_start && x.sediment_start == y.sediment_start &&
+           x.bed_change_start == y.bed_change_start && x.interval == y.interval && x.kenhhepd == y.kenhhepd &&
+           x.kenhhepng == y.kenhhepng && x.ketdinh == y.ketdinh && x.channel == y.channel && x.debug == y.debug &&
+           x.plot == y.plot && x.total_time == y.total_time;
+}
+
 }  // namespace
 
 struct hd_server {
     hd_server_config cfg{};
     std::mutex mu;
-    std::condition_variable wake, done;
+    std::condition_variable wake, done, create_cv;
     std::deque<Run *> queue;
-    std::deque<Run *> to_create;  // admitted, waiting for the helper thread
-    int creating = 0;             // admitted runs without a context yet
-    std::vector<Run *> active;
+    std::deque<Group *> to_create;  // admitted, waiting for the helper thread
+    std::vector<Group *> active;
+    int slots = 0;  // admitted runs whose group has not retired
     std::map<int64_t, std::unique_ptr<Run>> runs;
     size_t used = 0;
     int64_t next_id = 1;
     bool stop = false;
     std::thread worker, creator;
-    std::condition_variable create_cv;
     // statistics
     std::vector<double> latency;  // seconds, completed runs
     double cell_steps = 0;
     long submitted = 0, admitted = 0, completed = 0, failed = 0, cancelled = 0;
+    long batched = 0;  // runs admitted into a group of two or more
     Clock::time_point first_submit;
     Clock::time_point last_finish;
     bool any = false;
@@ -102,189 +141,224 @@ struct hd_server {
     void loop();
     void create_loop();
     bool admissible() const;
-    void finish(Run *r, int state);
+    void admit();
+    void retire(Group *g, int state);
 };
 
 // Whether the worker can act on the head of the queue now: a free slot and
 // a head that fits the budget, or one that leaves the queue anyway.  Called
 // with mu held.
 bool hd_server::admissible() const {
-    if (queue.empty() || (int)active.size() + creating >= cfg.max_active) return false;
+    if (queue.empty() || slots >= cfg.max_active) return false;
     const Run *r = queue.front();
     return r->cancel || r->footprint > cfg.hbm_budget || used + r->footprint <= cfg.hbm_budget;
 }
 
-void hd_server::finish(Run *r, int state) {
-    // called by the worker without the lock held
-    if (r->ctx) {
-        if (state == HD_RUN_DONE) {
-            const size_t n = (size_t)(r->mesh.M + 3) * (r->mesh.N + 3);
-            for (int k = 0; k < 5; k++) {
+// FIFO admission by HBM footprint and slot count: the queue's head, with
+// the queued runs that can share its launches, becomes a group for the
+// helper thread.  Called with mu held.
+void hd_server::admit() {
+    while (!queue.empty() && slots < cfg.max_active) {
+        Run *r = queue.front();
+        if (r->cancel) {
+            queue.pop_front();
+            r->state = HD_RUN_CANCELLED;
+            cancelled++;
+            done.notify_all();
+            continue;
+        }
+        if (r->footprint > cfg.hbm_budget) {
+            queue.pop_front();
+            r->state = HD_RUN_FAILED;
+            r->rc = HD_E_NOMEM;
+            r->err = "run needs " + std::to_string(r->footprint) + " bytes, over the server's HBM budget";
+            failed++;
+            done.notify_all();
+            continue;
+        }
+        if (used + r->footprint > cfg.hbm_budget) break;
+        std::vector<Run *> m{r};
+        for (size_t q = 1; q < queue.size() && (int)m.size() < kMaxBatch && slots + (int)m.size() < cfg.max_active;
+             q++)
+            if (!queue[q]->cancel && same_launches(*r, *queue[q])) m.push_back(queue[q]);
+        size_t fp = m.size() == 1 ? r->footprint : hd::footprint(&r->mesh, r->cfg.dense_layout, (int)m.size());
+        while (m.size() > 1 && used + fp > cfg.hbm_budget) {
+            m.pop_back();
+            fp = m.size() == 1 ? r->footprint : hd::footprint(&r->mesh, r->cfg.dense_layout, (int)m.size());
+        }
+        for (Run *o : m) queue.erase(std::find(queue.begin(), queue.end(), o));
+        Group *g = new Group();
+        g->members = m;
+        g->footprint = fp;
+        g->t = r->t;
+        g->T = r->T;
+        for (size_t k = 0; k < m.size(); k++) {
+            m[k]->group = g;
+            m[k]->scen = (int)k;
+        }
+        used += fp;
+        slots += (int)m.size();
+        admitted += (long)m.size();
+        if (m.size() > 1) batched += (long)m.size();
+        to_create.push_back(g);
+        create_cv.notify_one();
+    }
+}
+
+// Ends a group: copies its runs' results out when it completed, frees its
+// context and its budget, and records each run's end.  Called without mu
+// held, by the worker or the helper thread; the group is deleted.
+void hd_server::retire(Group *g, int state) {
+    if (g->ctx && state == HD_RUN_DONE) {
+        const size_t n = (size_t)(g->members[0]->mesh.M + 3) * (g->members[0]->mesh.N + 3);
+        for (Run *r : g->members) {
+            if (r->cancel) continue;
+            for (int k = 0; k < kNumResults && !r->rc; k++) {
                 r->results[k].resize(n);
-                int rc = hd_copy_out(r->ctx, kResultFields[k], r->results[k].data(), n);
-                if (rc && !r->rc) {
+                int rc = hd_batch_copy_out(g->ctx, r->scen, kResultFields[k], r->results[k].data(), n);
+                if (rc) {
                     r->rc = rc;
-                    r->err = hd_last_error(r->ctx);
+                    r->err = hd_last_error(g->ctx);
                 }
             }
-            std::vector<int32_t> kh(n);
-            int rc = hd_copy_out_i32(r->ctx, HD_F_KHOUOT, kh.data(), n);
-            if (rc && !r->rc) {
-                r->rc = rc;
-                r->err = hd_last_error(r->ctx);
-            }
-            r->results[5].assign(kh.begin(), kh.end());
-            if (r->rc) state = HD_RUN_FAILED;
         }
-        hd_destroy(r->ctx);
-        r->ctx = nullptr;
     }
+    if (g->ctx) hd_destroy(g->ctx);
+    g->ctx = nullptr;
     std::lock_guard<std::mutex> lk(mu);
-    r->state = state;
-    r->finished = Clock::now();
-    used -= r->footprint;
-    if (state == HD_RUN_DONE) {
-        completed++;
-        latency.push_back(std::chrono::duration<double>(r->finished - r->submitted).count());
-        last_finish = r->finished;
-    } else if (state == HD_RUN_CANCELLED) {
-        cancelled++;
-    } else {
-        failed++;
+    const Clock::time_point now = Clock::now();
+    for (Run *r : g->members) {
+        r->t = g->t;
+        r->steps = g->steps;
+        r->finished = now;
+        if (g->rc && !r->rc) {
+            r->rc = g->rc;
+            r->err = g->err;
+        }
+        int st = state;
+        if (r->cancel) st = HD_RUN_CANCELLED;
+        else if (st == HD_RUN_DONE && r->rc) st = HD_RUN_FAILED;
+        r->state = st;
+        r->group = nullptr;
+        if (st == HD_RUN_DONE) {
+            completed++;
+            latency.push_back(std::chrono::duration<double>(now - r->submitted).count());
+            last_finish = now;
+            cell_steps += (double)r->mesh.M * r->mesh.N * r->steps;
+        } else if (st == HD_RUN_CANCELLED) {
+            cancelled++;
+        } else {
+            failed++;
+        }
     }
+    used -= g->footprint;
+    slots -= (int)g->members.size();
+    auto it = std::find(active.begin(), active.end(), g);
+    if (it != active.end()) active.erase(it);
+    delete g;
     done.notify_all();
-    wake.notify_all();  // budget freed: the queue's head may fit now
+    wake.notify_all();  // budget and slots freed: the queue's head may fit now
 }
 
 // The helper thread: creates and initialises the context of each admitted
-// run in admission order, then hands the run to the worker.
+// group in admission order, then hands the group to the worker.
 void hd_server::create_loop() {
     (void)hipSetDevice(cfg.device);
     for (;;) {
-        Run *r;
+        Group *g;
         {
             std::unique_lock<std::mutex> lk(mu);
             create_cv.wait(lk, [&] { return stop || !to_create.empty(); });
             if (stop) return;
-            r = to_create.front();
+            g = to_create.front();
             to_create.pop_front();
         }
-        r->started = Clock::now();
-        int rc = hd_create(&r->mesh, &r->coeffs, &r->ops, &r->cfg, &r->ctx);
-        if (!rc) rc = hd_set_graph(r->ctx, cfg.use_graph);
-        if (!rc) rc = hd_init(r->ctx);
-        if (rc) {
-            r->rc = rc;
-            r->err = hd_last_error(r->ctx);
+        const Clock::time_point now = Clock::now();
+        std::vector<hd_mesh> meshes;
+        for (Run *r : g->members) {
+            r->started = now;
+            meshes.push_back(r->mesh);
         }
-        bool cancel;
+        const Run *r0 = g->members[0];
+        int rc = hd_batch_create(meshes.data(), (int)meshes.size(), &r0->coeffs, &r0->ops, &r0->cfg, &g->ctx);
+        if (!rc) rc = hd_set_graph(g->ctx, cfg.use_graph);
+        if (!rc) rc = hd_init(g->ctx);
+        if (rc) {
+            g->rc = rc;
+            g->err = hd_last_error(g->ctx);
+        }
+        bool all_cancelled = true;
         {
             std::lock_guard<std::mutex> lk(mu);
-            creating--;
-            cancel = r->cancel;
-            if (!rc && !cancel) {
-                r->state = HD_RUN_RUNNING;
-                active.push_back(r);
+            for (Run *r : g->members) all_cancelled = all_cancelled && r->cancel;
+            if (!rc && !all_cancelled) {
+                for (Run *r : g->members) r->state = HD_RUN_RUNNING;
+                active.push_back(g);
             }
             wake.notify_all();
         }
-        if (rc) finish(r, HD_RUN_FAILED);
-        else if (cancel) finish(r, HD_RUN_CANCELLED);
+        if (rc) retire(g, HD_RUN_FAILED);
+        else if (all_cancelled) retire(g, HD_RUN_CANCELLED);
     }
 }
 
 void hd_server::loop() {
     (void)hipSetDevice(cfg.device);
     for (;;) {
+        std::vector<Group *> cur;
         {
             std::unique_lock<std::mutex> lk(mu);
             wake.wait(lk, [&] { return stop || !active.empty() || admissible(); });
             if (stop) return;
-            // FIFO admission by HBM footprint and slot count; the contexts
-            // are made on the helper thread
-            while (!queue.empty() && (int)active.size() + creating < cfg.max_active) {
-                Run *r = queue.front();
-                if (r->cancel) {
-                    queue.pop_front();
-                    r->state = HD_RUN_CANCELLED;
-                    cancelled++;
-                    done.notify_all();
-                    continue;
-                }
-                if (r->footprint > cfg.hbm_budget) {
-                    queue.pop_front();
-                    r->state = HD_RUN_FAILED;
-                    r->rc = HD_E_NOMEM;
-                    r->err = "run needs " + std::to_string(r->footprint) + " bytes, over the server's HBM budget";
-                    failed++;
-                    done.notify_all();
-                    continue;
-                }
-                if (used + r->footprint > cfg.hbm_budget) break;
-                queue.pop_front();
-                used += r->footprint;
-                admitted++;
-                creating++;
-                to_create.push_back(r);
-                create_cv.notify_one();
-            }
-        }
-        std::vector<Run *> cur;
-        {
-            std::lock_guard<std::mutex> lk(mu);
+            admit();
             cur = active;
         }
-        // One quantum per run, each on its own stream.  The largest run
-        // takes steps_per_quantum steps and sets the round's length; a
-        // smaller run takes more steps in the same round, in inverse
-        // proportion to its cell count (up to kQuantumSpread times as
-        // many), so small runs finish and free their slots sooner instead
-        // of advancing in lockstep with the largest one.
+        // One quantum per group, each on its own stream.  The largest group
+        // (by cells over its runs) takes steps_per_quantum steps and sets
+        // the round's length; a smaller group takes more steps in the same
+        // round, in inverse proportion to its cell count (up to
+        // kQuantumSpread times as many), so small runs finish and free their
+        // slots sooner instead of advancing in lockstep with the largest.
         double cells_max = 0;
-        for (Run *r : cur) cells_max = std::max(cells_max, (double)r->mesh.M * r->mesh.N);
-        for (Run *r : cur) {
-            const double ratio = cells_max / ((double)r->mesh.M * r->mesh.N);
-            const int quantum = cfg.steps_per_quantum * std::max(1, std::min(kQuantumSpread, (int)ratio));
-            for (int k = 0; k < quantum && r->t < r->T && !r->rc; k++) {
-                int rc = hd_step(r->ctx, &r->t);
+        for (Group *g : cur) cells_max = std::max(cells_max, g->cells());
+        for (Group *g : cur) {
+            const int quantum =
+                cfg.steps_per_quantum * std::max(1, std::min(kQuantumSpread, (int)(cells_max / g->cells())));
+            for (int k = 0; k < quantum && g->t < g->T && !g->rc; k++) {
+                int rc = hd_step(g->ctx, &g->t);
                 if (rc) {
-                    r->rc = rc;
-                    r->err = hd_last_error(r->ctx);
+                    g->rc = rc;
+                    g->err = hd_last_error(g->ctx);
                 } else {
-                    r->steps++;
+                    g->steps++;
                 }
             }
         }
-        std::vector<std::pair<Run *, int>> retire;
-        for (Run *r : cur) {
-            if (!r->rc) {
-                int rc = hd_sync(r->ctx);
+        std::vector<std::pair<Group *, int>> ending;
+        for (Group *g : cur) {
+            if (!g->rc) {
+                int rc = hd_sync(g->ctx);
                 if (rc) {
-                    r->rc = rc;
-                    r->err = hd_last_error(r->ctx);
+                    g->rc = rc;
+                    g->err = hd_last_error(g->ctx);
                 }
             }
-            bool cancel;
+            bool all_cancelled = true;
             {
                 std::lock_guard<std::mutex> lk(mu);
-                cancel = r->cancel;
+                for (Run *r : g->members) {
+                    r->t = g->t;
+                    all_cancelled = all_cancelled && r->cancel;
+                }
             }
-            if (r->rc)
-                retire.push_back({r, HD_RUN_FAILED});
-            else if (cancel)
-                retire.push_back({r, HD_RUN_CANCELLED});
-            else if (r->t >= r->T)
-                retire.push_back({r, HD_RUN_DONE});
+            if (g->rc)
+                ending.push_back({g, HD_RUN_FAILED});
+            else if (all_cancelled)
+                ending.push_back({g, HD_RUN_CANCELLED});
+            else if (g->t >= g->T)
+                ending.push_back({g, HD_RUN_DONE});
         }
-        for (auto &rs : retire) {
-            {
-                std::lock_guard<std::mutex> lk(mu);
-                active.erase(std::find(active.begin(), active.end(), rs.first));
-                if (rs.second == HD_RUN_DONE)
-                    cell_steps += (double)rs.first->mesh.M * rs.first->mesh.N * rs.first->steps;
-            }
-            finish(rs.first, rs.second);
-        }
+        for (auto &e : ending) retire(e.first, e.second);
     }
 }
 
@@ -439,10 +513,10 @@ int hd_server_stats(hd_server *s, char *buf, size_t n) {
     int written = std::snprintf(
         buf, n,
         "{\"submitted\": %ld, \"admitted\": %ld, \"completed\": %ld, \"failed\": %ld, \"cancelled\": %ld, "
-        "\"queued\": %zu, \"active\": %zu, \"p50_s\": %.6f, \"p99_s\": %.6f, \"mean_s\": %.6f, "
+        "\"queued\": %zu, \"active\": %d, \"batched\": %ld, \"p50_s\": %.6f, \"p99_s\": %.6f, \"mean_s\": %.6f, "
         "\"cell_steps\": %.6e, \"wall_s\": %.6f, \"cell_steps_per_s\": %.6e, \"hbm_used\": %zu, "
         "\"hbm_budget\": %zu}",
-        s->submitted, s->admitted, s->completed, s->failed, s->cancelled, s->queue.size(), s->active.size(),
+        s->submitted, s->admitted, s->completed, s->failed, s->cancelled, s->queue.size(), s->slots, s->batched,
         pct(0.5), pct(0.99), mean, s->cell_steps, wall, busy > 0 ? s->cell_steps / busy : 0.0, s->used,
         s->cfg.hbm_budget);
     return written > 0 && (size_t)written < n ? HD_OK : HD_E_INVALID;
@@ -458,10 +532,13 @@ int hd_server_destroy(hd_server *s) {
     }
     if (s->worker.joinable()) s->worker.join();
     if (s->creator.joinable()) s->creator.join();
-    // runs the helper thread had made a context for but not yet handed over
-    // are in `active` too; runs still waiting for one have none
-    for (Run *r : s->active)
-        if (r->ctx) hd_destroy(r->ctx);
+    // groups the helper thread made a context for are in `active`; groups
+    // still waiting for one have none
+    for (Group *g : s->active) {
+        if (g->ctx) hd_destroy(g->ctx);
+        delete g;
+    }
+    for (Group *g : s->to_create) delete g;
     delete s;
     return HD_OK;
 }

@@ -376,11 +376,14 @@ int hd_memory_bytes(hd_ctx *ctx, size_t *bytes);
 /* process, main.cpp): what-if runs are submitted as they arrive; a    */
 /* worker thread admits them in order while their footprint fits the   */
 /* HBM budget and a slot is free, keeps each admitted run resident in  */
-/* its own context and stream (made on a helper thread), and advances  */
-/* all admitted runs together in rounds: the largest run takes         */
+/* a context and stream (made on a helper thread), and advances all    */
+/* admitted runs together in rounds: the largest takes                 */
 /* steps_per_quantum steps per round, a smaller one up to 4x as many,  */
 /* in inverse proportion to its cell count (their kernels overlap on   */
-/* the device).  A run's result does not depend on what else runs.     */
+/* the device).  Queued runs of one mesh size, duration, form, options */
+/* and coefficients are admitted together into one batched context     */
+/* (up to 16), so each launch serves all of them.  A run's result does */
+/* not depend on what else runs.                                       */
 /* ------------------------------------------------------------------ */
 typedef struct hd_server hd_server;
 
@@ -408,8 +411,11 @@ int hd_server_error(hd_server *s, int64_t id, char *buf, size_t n);
 int hd_server_cancel(hd_server *s, int64_t id);
 /* Drops a finished run's stored results. */
 int hd_server_release(hd_server *s, int64_t id);
-/* JSON: counts, p50/p99/mean completion latency (s, submit to results on
- * the host), cell-steps of completed runs and their rate. */
+/* JSON: counts (active: admitted runs not yet ended; batched: runs
+ * admitted into a batched context of two or more), p50/p99/mean
+ * completion latency (s, submit to results on the host), cell-steps of
+ * completed runs and their rate.  A cancelled run of a batched context
+ * ends when its context does. */
 int hd_server_stats(hd_server *s, char *buf, size_t n);
 int hd_server_destroy(hd_server *s);
 

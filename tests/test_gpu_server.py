@@ -92,3 +92,29 @@ def test_server_budget_and_cancel():
     stats = srv.stats()
     assert stats["failed"] == 1 and stats["cancelled"] >= 1, stats
     srv.close()
+
+
+def test_server_batches_runs_that_share_launches():
+    """Runs of one mesh size, duration, form and options (depths, islands
+    and hydrographs differ) that wait together are admitted as one batched
+    context: every launch serves all of them, and each result equals the
+    same run alone, bit for bit.  A larger run holds the budget first, so
+    the four small ones are all queued when it leaves."""
+    big = S.open_box(128, 96, seed=5)
+    small = [S.open_box(40, 28, seed=20 + k) for k in range(4)]
+    fp_big, fp_group = engine.footprint(big), engine.footprint(small[0], nscen=4)
+    assert fp_group <= fp_big  # the four fit together once the large run has left
+    _case("server batching", fp_big=fp_big, fp_group=fp_group)
+    srv = engine.Server(hbm_budget=fp_big + 1, max_active=8, steps_per_quantum=4, use_graph=False)
+    r_big = srv.submit(big, 10.0, form=FAST)
+    ids = [srv.submit(sc, 16.0, form=FAST, sediment=True, bed_change=True) for sc in small]
+    for rid in [r_big] + ids:
+        st = srv.wait(rid, timeout=120.0)
+        assert st == engine.RUN_DONE, (rid, st, srv.error(rid))
+    stats = srv.stats()
+    assert stats["batched"] == 4 and stats["completed"] == 5 and stats["failed"] == 0, stats
+    for rid, sc in zip(ids, small):
+        ref = alone(sc, 16.0, FAST, True)
+        for name in FIELDS:
+            assert bits_equal(srv.result(rid, name), ref[name]), (rid, sc.name, name)
+    srv.close()
