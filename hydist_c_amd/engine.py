@@ -404,7 +404,9 @@ def footprint(scen, nscen=1, dense_layout=0):
 
 class Server:
     """The serving loop: submit what-if runs as they arrive; they are
-    admitted by HBM footprint and advanced together on the GPU."""
+    admitted by HBM footprint and advanced together on the GPU.  Queued
+    runs of one mesh size, duration, form and options share one batched
+    context (stats()["batched"] counts them)."""
 
     def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_quantum=16, use_graph=True, device=0):
         L = _server_lib()
