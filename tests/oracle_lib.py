@@ -16,6 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libhydist_oracle.so")
 
 NATIVE_PATH = os.path.join(ROOT, "oracle", "_build", "native", "libhydist_oracle.so")
+SANITIZED_PATH = os.path.join(ROOT, "oracle", "_build", "sanitize", "libhydist_oracle.so")
 
 _lib = None
 
@@ -30,6 +31,15 @@ def cpu_model():
             if line.startswith("model name"):
                 return line.split(":", 1)[1].strip()
     return "unknown"
+
+
+def use_sanitized_build():
+    """Switches this process to the AddressSanitizer + UBSan copy of the
+    oracle (`make sanitize`); the process must have libasan preloaded."""
+    global LIB_PATH, _lib
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "sanitize"], check=True)
+    LIB_PATH = SANITIZED_PATH
+    _lib = None
 
 
 def use_native_build():
