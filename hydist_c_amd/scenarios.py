@@ -14,6 +14,8 @@ reference formulas take pow() of a negative face depth (Kx1/Ky1,
 support_funcs.cpp:64-67; Q-boundary weights, :509-514) or divide by a zero
 face depth, and produce NaN on the first step.
 """
+import json
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -521,3 +523,37 @@ def straight_channel(M=160, N=8, depth=4.0, manning=0.03, Q=-10.0, z_down=0.0, h
         bc=[_series(Q, total_time), _series(z_down, total_time), zeros, zeros], CC=None,
         meta={"Q": Q, "manning": manning, "depth": depth},
     )
+
+
+CONFIG_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs")
+
+
+def load_config(name):
+    """The benchmark configuration `name` (c1 ... c5, or the path of a JSON
+    file of the same form): the generator and its arguments, Load_coeffs
+    overrides (Constant_Coeffs member names), Options values (Tmax,
+    sediment_start, bed_change_start), the forms it is run in and its
+    sediment switches."""
+    path = name if os.path.exists(name) else os.path.join(CONFIG_DIR, f"{name.lower()}.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def config_scenario(cfg, k=0):
+    """Scenario k of a configuration (a load_config result): the named
+    generator called with the file's arguments (member k of an ensemble;
+    request k of the serving workload, which returns (scenario, Tmax, form,
+    sediment)), with the file's coefficient overrides and option values
+    applied."""
+    gen = globals()[cfg["generator"]]
+    args = {a: tuple(v) if isinstance(v, list) else v for a, v in cfg.get("args", {}).items()}
+    out = gen(k, **args) if cfg["generator"] in ("ensemble_member", "serving_request") else gen(**args)
+    sc = out[0] if isinstance(out, tuple) else out
+    sc.coeff_overrides = dict(sc.coeff_overrides, **cfg.get("coeffs", {}))
+    opts = cfg.get("options", {})
+    if "sediment_start" in opts:
+        sc.sediment_start = float(opts["sediment_start"])
+    if "bed_change_start" in opts:
+        sc.bed_change_start = float(opts["bed_change_start"])
+    return out
+
