@@ -34,6 +34,9 @@ def test_assembly_and_extraction_symbols_by_layout():
     assert bench.kernel_symbol("k_hyd_extract<uz>", SC, None, 1) == "k_hyd_extract_cols"
     assert bench.kernel_symbol("k_thomas<vz>", SC, None, 0) == "k_thomas"
     assert bench.kernel_symbol("k_post_a", SC, None, 1) == "k_post_a"
+    # the post passes that also form VTH, Kx, Ky run the same kernels
+    assert bench.kernel_symbol("k_post_a+vth", SC, None, 1) == "k_post_a"
+    assert bench.kernel_symbol("k_post_b+vth", SC, None, 1) == "k_post_b"
 
 
 def test_stage_bytes_all_positions():
@@ -49,6 +52,9 @@ def test_stage_bytes_all_positions():
     assert bench.stage_bytes("k_hyd_assemble<uz>", u, SC, all_positions=True) == 160.0 * pos
     cells = (SC.M + 3) * (SC.N + 3)
     assert bench.stage_bytes("k_segment", u, SC) == 8.0 * cells
+    # with VTH, Kx, Ky: three fields out and H_moi (and htaiz) in on top
+    assert bench.stage_bytes("k_post_a+vth", u, SC) == bench.stage_bytes("k_post_a", u, SC) + 32.0 * cells
+    assert bench.stage_bytes("k_post_b+vth", u, SC) == bench.stage_bytes("k_post_b", u, SC) + 40.0 * cells
     assert bench.stage_bytes("no_such_kernel", u, SC) is None
 
 
