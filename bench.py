@@ -575,6 +575,9 @@ def main():
         "ensemble": ensemble,
         "profile_top": sorted(({"stage": k, "launches": v[0], "avg_ms": v[1] / max(v[0], 1)} for k, v in stats.items()),
                               key=lambda d: -d["avg_ms"] * 1)[:8],
+        # every stage of the fastest form: launches and average ms per
+        # launch in the profile pass (HIP events on the library's stream)
+        "stages": {k: [v[0], round(v[1] / max(v[0], 1), 5)] for k, v in sorted(stats.items()) if v[0]},
     }
     print(json.dumps(out), flush=True)
 
