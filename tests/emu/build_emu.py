@@ -19,6 +19,11 @@ from hydist_c_amd import build as hb  # noqa: E402
 
 OUT = os.path.join(ROOT, "tests", "_build", "emu")
 LIB = os.path.join(OUT, "libhydist_emu.so")
+# the same with AddressSanitizer on the host code (the library's own C++:
+# contexts, the compat layer's loaders, the serving loop's threads); the
+# emulated device buffers are mmap'd and stay checked by their guard pages
+OUT_ASAN = os.path.join(ROOT, "tests", "_build", "emu_asan")
+ASAN = ["-fsanitize=address", "-fno-omit-frame-pointer"]
 LAUNCH = re.compile(r"([A-Za-z_][\w:]*(?:<[^<>;(){}]*>)?)\s*<<<(.*?)>>>\s*\(", re.S)
 
 
@@ -27,9 +32,9 @@ def transform(text):
     return LAUNCH.sub(lambda m: f'hd_emu::launcher("{m.group(1)}", {m.group(1)}, {m.group(2)})(', text)
 
 
-def _compile(args):
+def _compile(args, extra=()):
     src, obj = args
-    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-ffp-contract=off", "-w",
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-ffp-contract=off", "-w", *extra,
            "-I" + os.path.join(ROOT, "tests", "emu"), "-I" + os.path.join(ROOT, "include"), "-I" + hb.CSRC,
            "-x", "c++", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -38,8 +43,14 @@ def _compile(args):
     return obj
 
 
-def build(verbose=False):
-    src_dir = os.path.join(OUT, "src")
+def build(verbose=False, sanitize=False):
+    """Builds the emulated library and the compat driver against it into
+    tests/_build/emu (sanitize: with AddressSanitizer, into
+    tests/_build/emu_asan).  Returns the library's path."""
+    out = OUT_ASAN if sanitize else OUT
+    extra = ASAN if sanitize else []
+    lib = os.path.join(out, "libhydist_emu.so")
+    src_dir = os.path.join(out, "src")
     os.makedirs(src_dir, exist_ok=True)
     jobs = []
     for s in hb.SOURCES:
@@ -51,23 +62,23 @@ def build(verbose=False):
             text = transform(f.read())
         with open(dst, "w") as f:
             f.write(text)
-        jobs.append((dst, os.path.join(OUT, os.path.splitext(s)[0] + ".o")))
-    jobs.append((os.path.join(HERE, "emu_runtime.cpp"), os.path.join(OUT, "emu_runtime.o")))
+        jobs.append((dst, os.path.join(out, os.path.splitext(s)[0] + ".o")))
+    jobs.append((os.path.join(HERE, "emu_runtime.cpp"), os.path.join(out, "emu_runtime.o")))
     with ThreadPoolExecutor(max_workers=8) as ex:
-        objs = list(ex.map(_compile, jobs))
-    r = subprocess.run(["g++", "-shared", "-o", LIB] + objs, capture_output=True, text=True)
+        objs = list(ex.map(lambda j: _compile(j, extra), jobs))
+    r = subprocess.run(["g++", "-shared", *extra, "-o", lib] + objs, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-4000:])
-    drv = os.path.join(OUT, "compat_driver")
-    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", "-I" + os.path.join(ROOT, "tests", "emu"),
+    drv = os.path.join(out, "compat_driver")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-g", *extra, "-I" + os.path.join(ROOT, "tests", "emu"),
                         "-I" + os.path.join(ROOT, "include"),
                         os.path.join(ROOT, "tests", "compat", "compat_driver.cpp"), "-o", drv,
-                        "-L" + OUT, "-lhydist_emu", "-Wl,-rpath," + OUT], capture_output=True, text=True)
+                        "-L" + out, "-lhydist_emu", "-Wl,-rpath," + out], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-4000:])
     if verbose:
-        print("built", LIB, drv)
-    return LIB
+        print("built", lib, drv)
+    return lib
 
 
 if __name__ == "__main__":
