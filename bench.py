@@ -167,6 +167,8 @@ def kernel_symbol(stage, sc=None, chosen=None, layout=None):
                 kpt = -(-n // threads)
                 k = 2 if kpt <= 2 else 4 if kpt <= 4 else 8
             return f"{stage[:len(prefix) - 1]}<{t}, {il}, {k}>"
+    if stage.startswith("k_thomas_seg"):
+        return "k_thomas_seg"
     if stage.startswith("k_thomas"):
         return "k_thomas"
     if stage.endswith("+vth"):  # k_post_a / k_post_b with the VTH, K pass folded in
@@ -196,8 +198,10 @@ def time_form(sc, form, steps, warmup, profile_steps):
     # advance to the sediment switch-on, then the warm-up steps
     while sim.t < sc.sediment_start:
         sim.step()
-    # line solver per sweep (partition K or cyclic-reduction workgroup size)
-    tune = engine.autotune(sim) if form != 0 else None
+    # line solver per sweep (fast forms: partition K or cyclic-reduction
+    # workgroup size; plain form: Thomas per line or per system, the same
+    # bits either way)
+    tune = engine.autotune(sim)
     sim.step(warmup)
     sim.sync()
     t0 = time.perf_counter()

@@ -1088,7 +1088,10 @@ int hd_autotune(hd_ctx *ctx, char *report, size_t n) {
         const int param = ctx->solver_param[order[k][0]][order[k][1]];
         char buf[48];
         std::snprintf(buf, sizeof(buf), "%s\"%s/%d\"", k ? ", " : "",
-                      kind == HD_SOLVER_CR ? "cr" : (kind == HD_SOLVER_PARTITION ? "part" : "default"), param);
+                      kind == HD_SOLVER_CR ? "cr"
+                                           : (kind == HD_SOLVER_PARTITION ? "part"
+                                                                          : (kind == HD_SOLVER_SEGMENTS ? "seg" : "default")),
+                      param);
         r += buf;
     }
     r += "]}";
@@ -1103,10 +1106,11 @@ int hd_set_solver(hd_ctx *ctx, int dir, int hydraulic, int kind, int param) {
     if (kind == HD_SOLVER_DEFAULT) ok = param == 0;
     if (kind == HD_SOLVER_PARTITION) ok = param == 0 || param == 2 || param == 4 || param == 8;
     if (kind == HD_SOLVER_CR) ok = param >= 0 && param <= 512 && param % 64 == 0;
+    if (kind == HD_SOLVER_SEGMENTS) ok = param == 0 && ctx->cfg.form == HD_FORM_FP64_PLAIN;
     if (!ok)
         return set_err(ctx, HD_E_INVALID,
-                       "solver: DEFAULT with 0, PARTITION with K in {0, 2, 4, 8}, or CR with a workgroup size "
-                       "that is a multiple of 64 up to 512 (0 = default)");
+                       "solver: DEFAULT with 0, PARTITION with K in {0, 2, 4, 8}, CR with a workgroup size "
+                       "that is a multiple of 64 up to 512 (0 = default), or SEGMENTS with 0 (plain form only)");
     ctx->solver_kind[dir][hydraulic ? 1 : 0] = kind;
     ctx->solver_param[dir][hydraulic ? 1 : 0] = param;
     drop_graphs(ctx);

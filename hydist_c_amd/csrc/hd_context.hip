@@ -135,6 +135,8 @@ int alloc_shared(hd_ctx *ctx, int nscen) {
     B.sstride = nsys;
     A((void **)&B.sys0, nsys * nscen * sizeof(int));
     A((void **)&B.sysn, nsys * nscen * sizeof(int));
+    A((void **)&B.segl, nsys * nscen * sizeof(int));
+    A((void **)&B.segcnt, std::max((size_t)64, nscen * sizeof(int)));
     A((void **)&ctx->clock, 64);
     A((void **)&ctx->wd_flag, 64);
     A((void **)&ctx->ds, sizeof(DevState) * nscen);
@@ -167,7 +169,8 @@ size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
                                std::max((size_t)(M + 3) * round_up(rowsU, 64), (size_t)(N + 3) * round_up(rowsV, 64)));
     const size_t stride = (sw + 63) / 64 * 64;
     const size_t nsys = std::max(capV * (M + 3), capH * (N + 3));
-    b += 7 * stride * nscen * 8 + stride * nscen + 2 * nsys * nscen * 4 + 64 + 64 + sizeof(DevState) * nscen;
+    b += 7 * stride * nscen * 8 + stride * nscen + 3 * nsys * nscen * 4 + std::max((size_t)64, nscen * sizeof(int)) +
+         64 + 64 + sizeof(DevState) * nscen;
     return b;
 }
 

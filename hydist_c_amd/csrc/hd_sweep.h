@@ -22,6 +22,8 @@ struct SweepBufs {
     double *A, *B, *C, *D, *X, *Ap, *Bp;
     signed char *code;
     int *sys0, *sysn;
+    int *segl;    // plain form, one lane per system: the sweep's systems (index into sys0 / sysn)
+    int *segcnt;  // and their number, per scenario
     int ld;          // leading dimension of the active layout
     int ld_rm[2];    // [dir]: lines per row in ROW_MAJOR
     int ld_lm[2];    // [dir]: rows per line in LINE_MAJOR
@@ -31,7 +33,7 @@ struct SweepBufs {
         SweepBufs b = *this;
         const size_t o = (size_t)k * stride, so = (size_t)k * sstride;
         b.A += o; b.B += o; b.C += o; b.D += o; b.X += o; b.Ap += o; b.Bp += o; b.code += o;
-        b.sys0 += so; b.sysn += so;
+        b.sys0 += so; b.sysn += so; b.segl += so; b.segcnt += k;
         return b;
     }
 };

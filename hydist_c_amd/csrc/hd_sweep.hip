@@ -1619,6 +1619,157 @@ static bool solver_extracts(const hd_ctx *ctx, bool hyd) {
     return kind == HD_SOLVER_DEFAULT || kind == HD_SOLVER_PARTITION;
 }
 
+// ---------------------------------------------------------------------------
+// Thomas with one lane per system (plain form, HD_SOLVER_SEGMENTS).  The
+// systems of a line are independent (identity rows between them), and
+// k_thomas's walk along a line carries values only across rows outside
+// every system, which no system reads: its first row divides by its own
+// diagonal, its last row ignores the value after it.  Solving each system
+// on a lane of its own therefore gives the same bits, with the serial
+// chain cut from a line's length to its longest system's and every system
+// of the sweep in flight at once.  k_seg_list lists the sweep's systems
+// (index line * cap + k into sys0 / sysn) in line order, one workgroup
+// per scenario (a workgroup-wide prefix of per-thread counts); k_thomas_seg
+// then gives list entry e to lane e (and e + stride, ... when the grid is
+// smaller than the list).  A system's row place (first, interior, last,
+// only) follows from its row index, so the row-code bytes are not read.
+// ---------------------------------------------------------------------------
+constexpr int SEGLIST_THREADS = 1024;
+
+__global__ void __launch_bounds__(SEGLIST_THREADS) k_seg_list(const DevState *__restrict__ S, SweepBufs Bin, int dir,
+                                                             int start, int end) {
+    const SweepBufs B = Bin.scen(blockIdx.x);
+    const DevState &s = S[blockIdx.x];
+    const int cap = dir == 1 ? s.capV : s.capH;
+    const int *__restrict__ cnt = dir == 1 ? s.segV_n : s.segH_n;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int per = (end - start + 1 + nt - 1) / nt;  // consecutive lines per thread
+    const int l0 = start + tid * per, l1 = min(end + 1, l0 + per);
+    auto live = [&](size_t o) { return B.sys0[o] >= 0 && B.sysn[o] >= 0; };
+    int mine = 0;
+    for (int L = l0; L < l1; L++)
+        for (int k = 0; k < cnt[L]; k++) mine += live((size_t)L * cap + k) ? 1 : 0;
+    __shared__ int sc[SEGLIST_THREADS];
+    sc[tid] = mine;
+    __syncthreads();
+    for (int off = 1; off < nt; off <<= 1) {  // inclusive scan
+        const int v = tid >= off ? sc[tid - off] : 0;
+        __syncthreads();
+        sc[tid] += v;
+        __syncthreads();
+    }
+    int pos = sc[tid] - mine;
+    for (int L = l0; L < l1; L++)
+        for (int k = 0; k < cnt[L]; k++) {
+            const size_t o = (size_t)L * cap + k;
+            if (live(o)) B.segl[pos++] = (int)o;
+        }
+    if (tid == nt - 1) B.segcnt[0] = sc[tid];
+}
+
+__global__ void __launch_bounds__(64) k_thomas_seg(SweepBufs Bin, int cap) {
+    const SweepBufs B = Bin.scen(blockIdx.y);
+    const int total = B.segcnt[0], ld = B.ld, stride = gridDim.x * 64;
+    for (int e = blockIdx.x * 64 + threadIdx.x; e < total; e += stride) {
+        const int o = B.segl[e];
+        const int line = o / cap, r0 = B.sys0[o], sn = B.sysn[o], nr = sn + 1;
+        const double *__restrict__ A = B.A + line;
+        const double *__restrict__ Bv = B.B + line;
+        const double *__restrict__ C = B.C + line;
+        const double *__restrict__ D = B.D + line;
+        double *__restrict__ Ap = B.Ap + line;
+        double *__restrict__ Bp = B.Bp + line;
+        double *__restrict__ X = B.X + line;
+        auto at = [&](int q) { return (size_t)(r0 + min(max(q, 0), sn)) * ld; };
+        auto code_of = [&](int q) { return sn == 0 ? 4 : (q == 0 ? 1 : (q == sn ? 3 : 2)); };
+        auto load_row = [&](int q) {
+            const size_t w = at(q);
+            return ThomasRow{A[w], Bv[w], C[w], D[w], code_of(min(max(q, 0), sn))};
+        };
+        // forward, as k_thomas (TB rows of loads in flight ahead)
+        double ApP = 0, BpP = 0;
+        auto fwd = [&](const ThomasRow &w, int q) {
+            const bool first = w.code == 1 || w.code == 4;
+            const double den = first ? w.b : w.a * ApP + w.b;
+            const double num = first ? w.d : w.d - (w.a * BpP);
+            const double ap = (w.code == 1 || w.code == 2) ? -w.c / den : 0.0;
+            const double bp = num / den;
+            ApP = ap;
+            BpP = bp;
+            Ap[at(q)] = ap;
+            Bp[at(q)] = bp;
+        };
+        const int nfull = nr / TB;
+        ThomasRow cur[TB], nxt[TB];
+#pragma unroll
+        for (int q = 0; q < TB; q++) nxt[q] = load_row(q);
+        for (int blk = 0; blk < nfull; blk++) {
+#pragma unroll
+            for (int q = 0; q < TB; q++) cur[q] = nxt[q];
+#pragma unroll
+            for (int q = 0; q < TB; q++) nxt[q] = load_row((blk + 1) * TB + q);
+#pragma unroll
+            for (int q = 0; q < TB; q++) fwd(cur[q], blk * TB + q);
+        }
+#pragma unroll
+        for (int q = 0; q < TB; q++) {
+            if (nfull * TB + q >= nr) break;
+            fwd(nxt[q], nfull * TB + q);
+        }
+        // backward, as k_thomas
+        double xnext = 0;
+        auto bwd = [&](int q, double pa, double pb) {
+            const int code = code_of(q);
+            const double xv = (code == 3 || code == 4) ? pb : pb + (pa * xnext);
+            xnext = xv;
+            X[at(q)] = xv;
+        };
+        for (int q = sn; q >= nfull * TB; q--) bwd(q, Ap[at(q)], Bp[at(q)]);
+        double pa[TB], pb[TB], qa[TB], qb[TB];
+#pragma unroll
+        for (int u = 0; u < TB; u++) {
+            qa[u] = Ap[at(nfull * TB - 1 - u)];
+            qb[u] = Bp[at(nfull * TB - 1 - u)];
+        }
+        for (int blk = nfull - 1; blk >= 0; blk--) {
+#pragma unroll
+            for (int u = 0; u < TB; u++) {
+                pa[u] = qa[u];
+                pb[u] = qb[u];
+            }
+#pragma unroll
+            for (int u = 0; u < TB; u++) {
+                qa[u] = Ap[at(blk * TB - 1 - u)];
+                qb[u] = Bp[at(blk * TB - 1 - u)];
+            }
+#pragma unroll
+            for (int u = 0; u < TB; u++) bwd(blk * TB + TB - 1 - u, pa[u], pb[u]);
+        }
+    }
+}
+
+// The plain form's solve of one sweep: Thomas per line (k_thomas), or per
+// system (kind HD_SOLVER_SEGMENTS: k_seg_list + k_thomas_seg, 8 lanes per
+// line in the grid; lanes loop when a sweep has more systems).  B.ld is
+// the ROW_MAJOR leading dimension.
+template <int DIR>
+static int plain_solve(hd_ctx *ctx, const DevState *S, const DevState &s, const SweepBufs &B, int n, bool hyd,
+                       int start, int end, int nrows, int kind) {
+    hipStream_t st = ctx->stream;
+    const int lines = end - start + 1;
+    if (kind == HD_SOLVER_SEGMENTS) {
+        LAUNCH(ctx, "k_seg_list", k_seg_list<<<n, SEGLIST_THREADS, 0, st>>>(S, B, DIR, start, end));
+        const int waves = std::max(16, (lines * 8 + 63) / 64);
+        LAUNCH(ctx,
+               hyd ? (DIR ? "k_thomas_seg<uz>" : "k_thomas_seg<vz>") : (DIR ? "k_thomas_seg<fs_j>" : "k_thomas_seg<fs_i>"),
+               k_thomas_seg<<<dim3(waves, n), 64, 0, st>>>(B, DIR == 1 ? s.capV : s.capH));
+        return HD_OK;
+    }
+    LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
+           k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
+    return HD_OK;
+}
+
 template <int DIR, int LAYOUT>
 static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
                       double t, double s_start) {
@@ -1634,8 +1785,8 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
     if (rc) return rc;
     const int lines = end - start + 1;
     if (LAYOUT == ROW_MAJOR) {
-        LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
-               k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
+        rc = plain_solve<DIR>(ctx, S, s, B, n, hyd, start, end, nrows, ctx->solver_kind[DIR][hyd ? 1 : 0]);
+        if (rc) return rc;
     } else {
         const int h = hyd ? 1 : 0;
         const bool fused = DIR == 0 && solver_extracts(ctx, hyd);
@@ -1762,9 +1913,67 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     return HD_OK;
 }
 
+// The plain form's tuning: Thomas per line against Thomas per system on
+// the rows assembled from the current state.  Both write only sweep
+// scratch and give the same bits, so the choice changes no result.
+template <int DIR>
+static int tune_plain(hd_ctx *ctx, const DevState *S, const DevState &s, int n, bool hyd, int start, int end,
+                      double t, double s_start, std::string &report) {
+    hipStream_t st = ctx->stream;
+    SweepBufs B = ctx->sweep;
+    const int nrows = sweep_rows(s, DIR, hyd);
+    B.ld = B.ld_rm[DIR];
+    int rc = launch_assembly<DIR, ROW_MAJOR>(ctx, S, s, B, n, hyd, start, end, t, s_start, 0);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return HD_E_HIP;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return HD_E_HIP;
+    }
+    const bool prof = ctx->prof;
+    ctx->prof = false;  // candidates are timed here, not as stages
+    int best = HD_SOLVER_DEFAULT;
+    float best_ms = 0;
+    char buf[160];
+    for (int kind : {HD_SOLVER_DEFAULT, HD_SOLVER_SEGMENTS}) {
+        rc = plain_solve<DIR>(ctx, S, s, B, n, hyd, start, end, nrows, kind);  // warm-up
+        if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? HD_OK : HD_E_HIP;
+        for (int r = 0; r < 3 && !rc; r++) rc = plain_solve<DIR>(ctx, S, s, B, n, hyd, start, end, nrows, kind);
+        if (!rc) rc = hipEventRecord(e1, st) == hipSuccess ? HD_OK : HD_E_HIP;
+        if (!rc) rc = hipEventSynchronize(e1) == hipSuccess ? HD_OK : HD_E_HIP;
+        if (rc) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= 3;
+        std::snprintf(buf, sizeof(buf), "%s\"%s%s/thomas_%s\": %.4f", report.size() > 1 ? ", " : "", DIR ? "uz" : "vz",
+                      hyd ? "" : "_fs", kind == HD_SOLVER_SEGMENTS ? "seg" : "line", ms);
+        report += buf;
+        if (kind == HD_SOLVER_DEFAULT || ms < best_ms) {
+            best = kind;
+            best_ms = ms;
+        }
+    }
+    ctx->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->solver_kind[DIR][hyd ? 1 : 0] = best;
+    ctx->solver_param[DIR][hyd ? 1 : 0] = 0;
+    return HD_OK;
+}
+
 int tune_line_solvers(hd_ctx *ctx, const DevState *S, const DevState &s, int n, std::string &report) {
-    if (ctx->cfg.form == HD_FORM_FP64_PLAIN) return HD_OK;  // Thomas: nothing to tune
     const hd_options &o = ctx->ops;
+    if (ctx->cfg.form == HD_FORM_FP64_PLAIN) {
+        int rc = tune_plain<1>(ctx, S, s, n, true, 2, s.M, 0.0, 0.0, report);
+        if (!rc) rc = tune_plain<0>(ctx, S, s, n, true, 2, s.N, 0.0, 0.0, report);
+        if (!rc && ctx->cfg.sediment)
+            rc = tune_plain<1>(ctx, S, s, n, false, 3, s.M - 1, o.sediment_start, o.bed_change_start, report);
+        if (!rc && ctx->cfg.sediment)
+            rc = tune_plain<0>(ctx, S, s, n, false, 3, s.N - 1, o.sediment_start, o.bed_change_start, report);
+        return rc;
+    }
     int rc = tune_impl<1>(ctx, S, s, n, true, 2, s.M, 0.0, 0.0, report);
     if (!rc) rc = tune_impl<0>(ctx, S, s, n, true, 2, s.N, 0.0, 0.0, report);
     if (!rc && ctx->cfg.sediment) rc = tune_impl<1>(ctx, S, s, n, false, 3, s.M - 1, o.sediment_start, o.bed_change_start, report);

@@ -287,23 +287,30 @@ int hd_set_graph(hd_ctx *ctx, int on);
  * UVZSolver_multithread.cpp:19-43, and the unused SPIKE solver,
  * gtsv.cpp:88-193).  One workgroup solves one line:
  *   HD_SOLVER_PARTITION: each thread eliminates K consecutive unknowns in
- *     registers, the one interface unknown per thread is solved by cyclic
- *     reduction in LDS (param = K: 2, 4 or 8; lines up to 512*K unknowns);
+ *     registers, the one interface unknown per thread is solved by
+ *     parallel cyclic reduction in LDS (param = K: 2, 4 or 8; lines up to
+ *     512*K unknowns);
  *   HD_SOLVER_CR: cyclic reduction of the whole line in LDS (param = the
  *     workgroup size, a multiple of 64 up to 512; lines up to 8 x that).
  * HD_SOLVER_DEFAULT is the partition solver with K = 4 (K = 8 above 2048
  * unknowns).  The two solvers, and the partition solver at different K,
  * round differently (agreement ~1e-15 relative in FP64); the CR workgroup
  * size does not change any bit. */
-enum { HD_SOLVER_DEFAULT = 0, HD_SOLVER_PARTITION = 1, HD_SOLVER_CR = 2 };
+/* The plain form always solves with Thomas in the reference's order
+ * (tridiag, UVZSolver_multithread.cpp:19-43): by default one lane walks a
+ * whole line, its systems one after the other; HD_SOLVER_SEGMENTS
+ * (param 0, plain form only) gives every system of the sweep a lane of
+ * its own.  Both give the same bits. */
+enum { HD_SOLVER_DEFAULT = 0, HD_SOLVER_PARTITION = 1, HD_SOLVER_CR = 2, HD_SOLVER_SEGMENTS = 3 };
 
 /* Autotuning (no counterpart in the reference).  Times each line solver
- * candidate (partition at K = 2, 4, 8; cyclic reduction at 128, 256, 512
- * threads) on every sweep (UZ, VZ, and the FS sweeps when sediment is on)
- * from the context's current state, keeps the fastest per sweep, and
- * writes a JSON report (ms per candidate and the choices) into report.
- * The choice can change results at rounding level (see above); it is
- * deterministic once made.  hd_set_solver sets one choice by hand
+ * candidate (fast forms: partition at K = 2, 4, 8, cyclic reduction at
+ * 128, 256, 512 threads; plain form: Thomas per line or per system) on
+ * every sweep (UZ, VZ, and the FS sweeps when sediment is on) from the
+ * context's current state, keeps the fastest per sweep, and writes a JSON
+ * report (ms per candidate and the choices) into report.  In the fast
+ * forms the choice can change results at rounding level (see above); it
+ * is deterministic once made.  In the plain form it changes no bit.  hd_set_solver sets one choice by hand
  * (dir 1 = UZ, 0 = VZ; hydraulic 1 = the z/u sweep, 0 = the FS sweep);
  * hd_set_solver_threads(ctx, dir, hyd, t) is hd_set_solver(ctx, dir, hyd,
  * HD_SOLVER_CR, t). */

@@ -33,6 +33,7 @@ def test_assembly_and_extraction_symbols_by_layout():
     assert bench.kernel_symbol("k_fs_extract<i>", SC, None, 0) == "k_fs_extract<0, 0>"
     assert bench.kernel_symbol("k_hyd_extract<uz>", SC, None, 1) == "k_hyd_extract_cols"
     assert bench.kernel_symbol("k_thomas<vz>", SC, None, 0) == "k_thomas"
+    assert bench.kernel_symbol("k_thomas_seg<fs_j>", SC, None, 0) == "k_thomas_seg"
     assert bench.kernel_symbol("k_post_a", SC, None, 1) == "k_post_a"
     # the post passes that also form VTH, Kx, Ky run the same kernels
     assert bench.kernel_symbol("k_post_a+vth", SC, None, 1) == "k_post_a"

@@ -1,0 +1,80 @@
+"""The plain form's Thomas with one lane per system (HD_SOLVER_SEGMENTS)
+against one lane per line and against the oracle: the same bits on every
+field (the systems of a line are independent; the line walk carries
+values only across rows no system reads), on meshes with many systems
+per line, with one- and two-cell systems, drying fronts, every boundary
+type and the sediment sweeps.  The tuner picks between the two by time
+alone, so a tuned run equals an untuned one bit for bit."""
+import numpy as np
+import pytest
+
+from hydist_c_amd import abi, engine, scenarios as S
+from parity_util import HYDRO, bits_equal, make_pair
+from test_gpu_edge_meshes import MESHES
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = HYDRO + ["KHOUOT", "FS", "DH"]
+CASES = {
+    "box": (lambda: S.open_box(40, 28), 12),
+    "box_z_q": (lambda: S.open_box(57, 33, seed=9, types=(0, 2, 0, 2)), 12),
+    "flats": (lambda: S.tidal_flats(64, 40, seed=6), 20),
+    "islands": (lambda: S.meander_islands(160, 96, n_islands=8), 8),
+    "speckled35": (MESHES["speckled35"], 6),
+    "stripes": (MESHES["stripes"], 6),
+    "onecell": (MESHES["onecell"], 6),
+}
+
+
+def per_system(sim):
+    for d in (0, 1):
+        for hyd in (0, 1):
+            engine.set_solver(sim, d, hyd, engine.SOLVER_SEGMENTS)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_thomas_per_system_matches_per_line_and_oracle(name):
+    maker, steps = CASES[name]
+    sc = maker()
+    sc.sediment_start = sc.bed_change_start = 0.0
+    print(f"[case] thomas per system {sc.name} steps={steps}", flush=True)
+    g, o = make_pair(sc, sediment=True, bed_change=True)
+    per_system(g)
+    line = engine.Simulation(sc, sediment=True, bed_change=True)
+    line.init()
+    for sim in (g, o, line):
+        sim.step(steps)
+    for f in HYDRO + ["KHOUOT"]:
+        assert bits_equal(g.get(f), o.get(f)), f
+    for f in FIELDS:
+        assert bits_equal(g.get(f), line.get(f)), f
+    g.close()
+    line.close()
+
+
+def test_plain_tuning_changes_no_bit():
+    sc = S.meander_islands(160, 96, n_islands=8)
+    sc.sediment_start = sc.bed_change_start = 0.0
+    print("[case] plain autotune", flush=True)
+    a = engine.Simulation(sc, sediment=True, bed_change=True)
+    b = engine.Simulation(sc, sediment=True, bed_change=True)
+    a.init()
+    b.init()
+    a.step(2)
+    b.step(2)
+    rep = engine.autotune(a)
+    assert any(k.endswith("thomas_seg") for k in rep) and any(k.endswith("thomas_line") for k in rep), rep
+    assert all(c in ("default/0", "seg/0") for c in rep["chosen"]), rep
+    a.step(6)
+    b.step(6)
+    for f in FIELDS:
+        assert bits_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
+
+
+def test_per_system_thomas_is_the_plain_forms():
+    sim = engine.Simulation(S.open_box(24, 18), form=abi.HD_FORM_FP64_FAST)
+    with pytest.raises(engine.HydistError):
+        engine.set_solver(sim, 0, 1, engine.SOLVER_SEGMENTS)
+    sim.close()
