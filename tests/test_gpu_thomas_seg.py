@@ -78,3 +78,26 @@ def test_per_system_thomas_is_the_plain_forms():
     with pytest.raises(engine.HydistError):
         engine.set_solver(sim, 0, 1, engine.SOLVER_SEGMENTS)
     sim.close()
+
+
+def test_thomas_per_system_in_a_batch_with_graphs():
+    """Batched scenarios (the list is made per scenario) under whole-step
+    graphs: every member equals its single per-line run bit for bit."""
+    scens = []
+    for k in range(3):
+        s = S.ensemble_member(k, M=96, N=48)
+        s.sediment_start = s.bed_change_start = 10.0
+        scens.append(s)
+    print("[case] thomas per system, batch of 3 with graphs", flush=True)
+    b = engine.Batch(scens, sediment=True, bed_change=True, graph=True)
+    per_system(b)
+    b.init()
+    b.step(12)
+    for k, sc in enumerate(scens):
+        g = engine.Simulation(sc, sediment=True, bed_change=True)
+        g.init()
+        g.step(12)
+        for f in ["U", "V", "Z", "FS", "DH", "KHOUOT"]:
+            assert bits_equal(b.get(k, f), g.get(f).astype(np.float64)), (k, f)
+        g.close()
+    b.close()
