@@ -1106,11 +1106,12 @@ int hd_set_solver(hd_ctx *ctx, int dir, int hydraulic, int kind, int param) {
     if (kind == HD_SOLVER_DEFAULT) ok = param == 0;
     if (kind == HD_SOLVER_PARTITION) ok = param == 0 || param == 2 || param == 4 || param == 8;
     if (kind == HD_SOLVER_CR) ok = param >= 0 && param <= 512 && param % 64 == 0;
-    if (kind == HD_SOLVER_SEGMENTS) ok = param == 0 && ctx->cfg.form == HD_FORM_FP64_PLAIN;
+    if (kind == HD_SOLVER_SEGMENTS)
+        ok = param == 0 && (ctx->cfg.form == HD_FORM_FP64_PLAIN || ctx->cfg.form == HD_FORM_FP64_FAST);
     if (!ok)
         return set_err(ctx, HD_E_INVALID,
                        "solver: DEFAULT with 0, PARTITION with K in {0, 2, 4, 8}, CR with a workgroup size "
-                       "that is a multiple of 64 up to 512 (0 = default), or SEGMENTS with 0 (plain form only)");
+                       "that is a multiple of 64 up to 512 (0 = default), or SEGMENTS with 0 (the FP64 forms)");
     ctx->solver_kind[dir][hydraulic ? 1 : 0] = kind;
     ctx->solver_param[dir][hydraulic ? 1 : 0] = param;
     drop_graphs(ctx);

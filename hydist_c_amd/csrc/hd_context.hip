@@ -34,7 +34,6 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     s.N = mesh->N;
     s.P = ctx->cfg.dense_layout ? s.M + 3 : round_up(s.M + 3, 32);
     s.H = s.N + 4;
-    s.ldT = round_up(s.N + 3, 32);
     s.capH = (s.M + 3) / 2 + 2;
     s.capV = (s.N + 3) / 2 + 2;
     s.total_time = mesh->total_time;
@@ -60,6 +59,7 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     F(&s.htaiz); F(&s.htaiz_bd); F(&s.Kx1); F(&s.Ky1);
     F(&s.u); F(&s.v); F(&s.z); F(&s.t_u); F(&s.t_v); F(&s.t_z); F(&s.H_moi); F(&s.Htdu); F(&s.Htdv);
     F(&s.FS); F(&s.tFS); F(&s.VTH); F(&s.Kx); F(&s.Ky); F(&s.Qbx); F(&s.Qby); F(&s.dH); F(&s.tmp);
+    F(&s.keep0); F(&s.keep1);
     A((void **)&s.khouot, cells * sizeof(int));
     A((void **)&s.segidH, cells * sizeof(int));
     A((void **)&s.segidV, cells * sizeof(int));
@@ -83,9 +83,6 @@ int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s) {
     A((void **)&s.segV_n, (s.M + 3) * sizeof(int));
     A((void **)&s.segV_f, (size_t)(s.M + 3) * s.capV * sizeof(int));
     A((void **)&s.segV_l, (size_t)(s.M + 3) * s.capV * sizeof(int));
-    const size_t scratch = std::max((size_t)2 * s.H * s.P, (size_t)2 * (s.M + 4) * s.ldT);
-    A((void **)&s.Ap, scratch * sizeof(double));
-    A((void **)&s.Bp, scratch * sizeof(double));
     A((void **)&s.acc, (8 + (size_t)s.H) * sizeof(double));
     A((void **)&s.rwmask, (size_t)((s.N >> 6) + 1) * s.P * sizeof(unsigned long long));
     if (rc) return rc;
@@ -154,15 +151,14 @@ int alloc_shared(hd_ctx *ctx, int nscen) {
 size_t footprint(const hd_mesh *mesh, int dense, int nscen) {
     const int M = mesh->M, N = mesh->N;
     const size_t P = dense ? M + 3 : round_up(M + 3, 32), H = N + 4;
-    const size_t ldT = round_up(N + 3, 32), capH = (M + 3) / 2 + 2, capV = (N + 3) / 2 + 2;
+    const size_t capH = (M + 3) / 2 + 2, capV = (N + 3) / 2 + 2;
     const size_t cells = P * H;
     auto al = [](size_t b) { return b ? b : (size_t)8; };
-    size_t b = 28 * cells * 8 + 5 * cells * 4;
+    size_t b = 30 * cells * 8 + 5 * cells * 4;
     b += al(2 * (size_t)(M + N + 6) * 8) + 32 + 2 * (M + 3) * 8 + 2 * (N + 3) * 8;
     b += 8 * al((size_t)mesh->total_time * 8);
     b += (N + 3) * 4 + 2 * (N + 3) * capH * 4 + (M + 3) * 4 + 2 * (M + 3) * capV * 4;
-    const size_t scratch = std::max(2 * H * P, 2 * (size_t)(M + 4) * ldT);
-    b += 2 * scratch * 8 + (8 + H) * 8 + (size_t)((N >> 6) + 1) * P * 8;
+    b += (8 + H) * 8 + (size_t)((N >> 6) + 1) * P * 8;
     b *= nscen;
     const int rowsU = 2 * (N - 1), rowsV = 2 * (M - 1);
     const size_t sw = std::max(std::max((size_t)rowsU * round_up(M + 3, 64), (size_t)rowsV * round_up(N + 3, 64)),

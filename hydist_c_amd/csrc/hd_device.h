@@ -21,7 +21,6 @@ namespace hd {
 struct DevState {
     int M, N, P, H;        // H = allocated rows (N + 4)
     int capH, capV;        // per-line capacity of the segment tables
-    int ldT;               // leading dimension of the row-sweep scratch (>= N + 3)
     int total_time;
     int btype[4], bienQ[4];
     int hasCC[4];
@@ -59,7 +58,9 @@ struct DevState {
     int *segflV;
     // scratch
     double *tmp;       // smoothing output (full field)
-    double *Ap, *Bp;   // Thomas sweep scratch, two rows per cell
+    // the line-solver tuner's copies of the fields a row-line sweep writes
+    // (t_z and t_v, or tFS), kept while it times whole sweeps
+    double *keep0, *keep1;
     double *acc;       // [0] boundary inflow volume, [1] mass-balance result
     // re-wetting candidates: bit (i & 63) of word [(i >> 6) * P + j] marks
     // row i of column j; cleared by the re-wetting pass that reads it

@@ -1589,9 +1589,16 @@ static const char *solver_name(int kind, bool f32, int dir, bool hyd) {
     return names[kind == HD_SOLVER_CR][f32][dir][hyd];
 }
 
+template <int LAYOUT>
+static int seg_solve(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int n, int dir, bool hyd, int start,
+                     int end);
+
 static int run_solver(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int nscen, int dir, bool hyd, int kind,
                       int param, int start, int end, int nrows, bool extract = false, bool assemble = false,
                       double t = 0.0, double s_start = 0.0) {
+    // FP64 Thomas with one lane per system (the fast FP64 form's extra
+    // candidate; it reads the systems' rows as assembled, in FP64)
+    if (kind == HD_SOLVER_SEGMENTS) return seg_solve<LINE_MAJOR>(ctx, S, B, nscen, dir, hyd, start, end);
     const int nz = hyd ? nrows / 2 : nrows;
     if (kind == HD_SOLVER_DEFAULT) kind = HD_SOLVER_PARTITION;
     int threads, kpt;
@@ -1633,6 +1640,9 @@ static bool solver_extracts(const hd_ctx *ctx, bool hyd) {
 // then gives list entry e to lane e (and e + stride, ... when the grid is
 // smaller than the list).  A system's row place (first, interior, last,
 // only) follows from its row index, so the row-code bytes are not read.
+// LAYOUT: ROW_MAJOR (the plain form's [rho][line]) or LINE_MAJOR (the
+// fast FP64 form's [line][rho], where the same FP64 Thomas is one more
+// candidate of the tuner).
 // ---------------------------------------------------------------------------
 constexpr int SEGLIST_THREADS = 1024;
 
@@ -1667,20 +1677,25 @@ __global__ void __launch_bounds__(SEGLIST_THREADS) k_seg_list(const DevState *__
     if (tid == nt - 1) B.segcnt[0] = sc[tid];
 }
 
-__global__ void __launch_bounds__(64) k_thomas_seg(SweepBufs Bin, int cap) {
+template <int LAYOUT>
+__global__ void __launch_bounds__(64) k_thomas_seg(SweepBufs Bin, const DevState *__restrict__ S, int dir) {
     const SweepBufs B = Bin.scen(blockIdx.y);
+    const int cap = dir == 1 ? S[blockIdx.y].capV : S[blockIdx.y].capH;
     const int total = B.segcnt[0], ld = B.ld, stride = gridDim.x * 64;
+    // element (rho, line) of the layout: base + line * lstep + rho * rstep
+    const size_t lstep = LAYOUT == ROW_MAJOR ? 1 : (size_t)ld, rstep = LAYOUT == ROW_MAJOR ? (size_t)ld : 1;
     for (int e = blockIdx.x * 64 + threadIdx.x; e < total; e += stride) {
         const int o = B.segl[e];
         const int line = o / cap, r0 = B.sys0[o], sn = B.sysn[o], nr = sn + 1;
-        const double *__restrict__ A = B.A + line;
-        const double *__restrict__ Bv = B.B + line;
-        const double *__restrict__ C = B.C + line;
-        const double *__restrict__ D = B.D + line;
-        double *__restrict__ Ap = B.Ap + line;
-        double *__restrict__ Bp = B.Bp + line;
-        double *__restrict__ X = B.X + line;
-        auto at = [&](int q) { return (size_t)(r0 + min(max(q, 0), sn)) * ld; };
+        const size_t lo = (size_t)line * lstep;
+        const double *__restrict__ A = B.A + lo;
+        const double *__restrict__ Bv = B.B + lo;
+        const double *__restrict__ C = B.C + lo;
+        const double *__restrict__ D = B.D + lo;
+        double *__restrict__ Ap = B.Ap + lo;
+        double *__restrict__ Bp = B.Bp + lo;
+        double *__restrict__ X = B.X + lo;
+        auto at = [&](int q) { return (size_t)(r0 + min(max(q, 0), sn)) * rstep; };
         auto code_of = [&](int q) { return sn == 0 ? 4 : (q == 0 ? 1 : (q == sn ? 3 : 2)); };
         auto load_row = [&](int q) {
             const size_t w = at(q);
@@ -1748,23 +1763,31 @@ __global__ void __launch_bounds__(64) k_thomas_seg(SweepBufs Bin, int cap) {
     }
 }
 
+// Thomas with one lane per system over a sweep's assembled systems in
+// LAYOUT (8 lanes per line in the grid; lanes loop when a sweep has more
+// systems).  B.ld is the layout's leading dimension.
+template <int LAYOUT>
+static int seg_solve(hd_ctx *ctx, const DevState *S, const SweepBufs &B, int n, int dir, bool hyd, int start,
+                     int end) {
+    hipStream_t st = ctx->stream;
+    const int lines = end - start + 1;
+    LAUNCH(ctx, "k_seg_list", k_seg_list<<<n, SEGLIST_THREADS, 0, st>>>(S, B, dir, start, end));
+    const int waves = std::max(16, (lines * 8 + 63) / 64);
+    LAUNCH(ctx,
+           hyd ? (dir ? "k_thomas_seg<uz>" : "k_thomas_seg<vz>") : (dir ? "k_thomas_seg<fs_j>" : "k_thomas_seg<fs_i>"),
+           k_thomas_seg<LAYOUT><<<dim3(waves, n), 64, 0, st>>>(B, S, dir));
+    return HD_OK;
+}
+
 // The plain form's solve of one sweep: Thomas per line (k_thomas), or per
-// system (kind HD_SOLVER_SEGMENTS: k_seg_list + k_thomas_seg, 8 lanes per
-// line in the grid; lanes loop when a sweep has more systems).  B.ld is
-// the ROW_MAJOR leading dimension.
+// system (kind HD_SOLVER_SEGMENTS).  B.ld is the ROW_MAJOR leading
+// dimension.
 template <int DIR>
 static int plain_solve(hd_ctx *ctx, const DevState *S, const DevState &s, const SweepBufs &B, int n, bool hyd,
                        int start, int end, int nrows, int kind) {
     hipStream_t st = ctx->stream;
     const int lines = end - start + 1;
-    if (kind == HD_SOLVER_SEGMENTS) {
-        LAUNCH(ctx, "k_seg_list", k_seg_list<<<n, SEGLIST_THREADS, 0, st>>>(S, B, DIR, start, end));
-        const int waves = std::max(16, (lines * 8 + 63) / 64);
-        LAUNCH(ctx,
-               hyd ? (DIR ? "k_thomas_seg<uz>" : "k_thomas_seg<vz>") : (DIR ? "k_thomas_seg<fs_j>" : "k_thomas_seg<fs_i>"),
-               k_thomas_seg<<<dim3(waves, n), 64, 0, st>>>(B, DIR == 1 ? s.capV : s.capH));
-        return HD_OK;
-    }
+    if (kind == HD_SOLVER_SEGMENTS) return seg_solve<ROW_MAJOR>(ctx, S, B, n, DIR, hyd, start, end);
     LAUNCH(ctx, hyd ? (DIR ? "k_thomas<uz>" : "k_thomas<vz>") : (DIR ? "k_thomas<fs_j>" : "k_thomas<fs_i>"),
            k_thomas<<<dim3((lines + 63) / 64, n), 64, 0, st>>>(B, start, end, nrows));
     return HD_OK;
@@ -1815,15 +1838,13 @@ static int sweep_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, 
 }
 
 // Copies the fields a row-line sweep writes (t_z and t_v, or tFS) of every
-// scenario into the Thomas scratch (unused by the fast forms), or back.
-// The caller checks that a field fits in one scenario's scratch.
+// scenario into its keep0 / keep1, or back.
 static int swap_sweep_outputs(hd_ctx *ctx, int n, bool hyd, bool save) {
-    const SweepBufs &B = ctx->sweep;
     for (int k = 0; k < n; k++) {
         const DevState &sk = k == 0 ? ctx->hs : ctx->hss[k - 1];
         const size_t bytes = (size_t)sk.P * sk.H * sizeof(double);
         double *f0 = hyd ? sk.t_z : sk.tFS, *f1 = sk.t_v;
-        double *a = B.Ap + (size_t)k * B.stride, *b = B.Bp + (size_t)k * B.stride;
+        double *a = sk.keep0, *b = sk.keep1;
         if (hipMemcpyAsync(save ? a : f0, save ? f0 : a, bytes, hipMemcpyDeviceToDevice, ctx->stream) != hipSuccess)
             return HD_E_HIP;
         if (hyd && hipMemcpyAsync(save ? b : f1, save ? f1 : b, bytes, hipMemcpyDeviceToDevice, ctx->stream) !=
@@ -1847,9 +1868,7 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     SweepBufs B = ctx->sweep;
     const int nrows = sweep_rows(s, DIR, hyd);
     B.ld = B.ld_lm[DIR];
-    // (a field fits the scratch on all but the smallest meshes, where the
-    // solve alone is timed)
-    const bool whole = DIR == 0 && (size_t)s.P * s.H <= B.stride;
+    const bool whole = DIR == 0;
     int arc = whole ? swap_sweep_outputs(ctx, n, hyd, true)
                     : launch_assembly<DIR, LINE_MAJOR>(ctx, S, s, B, n, hyd, start, end, t, s_start, 0);
     if (arc) return arc;
@@ -1873,11 +1892,17 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
     int best_kind = 0, best_param = 0, rc = HD_OK;
     float best_ms = 0;
     char buf[160];
-    const int cands[6][2] = {{HD_SOLVER_PARTITION, 2}, {HD_SOLVER_PARTITION, 4}, {HD_SOLVER_PARTITION, 8},
-                             {HD_SOLVER_CR, 128},      {HD_SOLVER_CR, 256},      {HD_SOLVER_CR, 512}};
+    // (Thomas per system, FP64, only for the fast FP64 form)
+    const int cands[7][2] = {{HD_SOLVER_PARTITION, 2}, {HD_SOLVER_PARTITION, 4}, {HD_SOLVER_PARTITION, 8},
+                             {HD_SOLVER_CR, 128},      {HD_SOLVER_CR, 256},      {HD_SOLVER_CR, 512},
+                             {HD_SOLVER_SEGMENTS, 0}};
     for (const auto &c : cands) {
         int threads, kpt;
-        if (!solver_shape(ctx, nz, c[0], c[1], threads, kpt)) continue;  // too long for this candidate
+        if (c[0] == HD_SOLVER_SEGMENTS) {
+            if (ctx->cfg.form != HD_FORM_FP64_FAST) continue;
+        } else if (!solver_shape(ctx, nz, c[0], c[1], threads, kpt)) {
+            continue;  // too long for this candidate
+        }
         rc = candidate(c[0], c[1]);  // warm-up
         if (!rc) rc = hipEventRecord(e0, st) == hipSuccess ? HD_OK : HD_E_HIP;
         for (int r = 0; r < 3 && !rc; r++) rc = candidate(c[0], c[1]);
@@ -1888,7 +1913,8 @@ static int tune_impl(hd_ctx *ctx, const DevState *S, const DevState &s, int n, b
         (void)hipEventElapsedTime(&ms, e0, e1);
         ms /= 3;
         std::snprintf(buf, sizeof(buf), "%s\"%s%s/%s%d\": %.4f", report.size() > 1 ? ", " : "", DIR ? "uz" : "vz",
-                      hyd ? "" : "_fs", c[0] == HD_SOLVER_CR ? "cr" : "part", c[1], ms);
+                      hyd ? "" : "_fs", c[0] == HD_SOLVER_CR ? "cr" : (c[0] == HD_SOLVER_SEGMENTS ? "seg" : "part"),
+                      c[1], ms);
         report += buf;
         if (!best_kind || ms < best_ms) {
             best_kind = c[0];

@@ -266,8 +266,9 @@ SOLVER_DEFAULT, SOLVER_PARTITION, SOLVER_CR, SOLVER_SEGMENTS = 0, 1, 2, 3
 def set_solver(sim, direction, hydraulic, kind, param=0):
     """Line solver of one sweep: SOLVER_PARTITION with K unknowns per thread
     (2, 4, 8; 0 = default), SOLVER_CR with a workgroup size, or
-    SOLVER_DEFAULT; in the plain form SOLVER_SEGMENTS (Thomas with one lane
-    per system; the same bits as the default, one lane per line)."""
+    SOLVER_DEFAULT; in the FP64 forms SOLVER_SEGMENTS (Thomas with one lane
+    per system: in the plain form the same bits as the default, one lane
+    per line; in the fast FP64 form one more choice at rounding level)."""
     check(lib().hd_set_solver(sim.h, direction, int(hydraulic), kind, param), sim.h)
 
 

@@ -299,13 +299,16 @@ int hd_set_graph(hd_ctx *ctx, int on);
 /* The plain form always solves with Thomas in the reference's order
  * (tridiag, UVZSolver_multithread.cpp:19-43): by default one lane walks a
  * whole line, its systems one after the other; HD_SOLVER_SEGMENTS
- * (param 0, plain form only) gives every system of the sweep a lane of
- * its own.  Both give the same bits. */
+ * (param 0) gives every system of the sweep a lane of its own.  Both give
+ * the same bits.  The fast FP64 form takes HD_SOLVER_SEGMENTS too: the
+ * same FP64 Thomas on the fast form's assembled systems, one more choice
+ * rounding differently from its other solvers (FP32 + refine: rejected). */
 enum { HD_SOLVER_DEFAULT = 0, HD_SOLVER_PARTITION = 1, HD_SOLVER_CR = 2, HD_SOLVER_SEGMENTS = 3 };
 
 /* Autotuning (no counterpart in the reference).  Times each line solver
  * candidate (fast forms: partition at K = 2, 4, 8, cyclic reduction at
- * 128, 256, 512 threads; plain form: Thomas per line or per system) on
+ * 128, 256, 512 threads, and in the fast FP64 form Thomas per system;
+ * plain form: Thomas per line or per system) on
  * every sweep (UZ, VZ, and the FS sweeps when sediment is on) from the
  * context's current state, keeps the fastest per sweep, and writes a JSON
  * report (ms per candidate and the choices) into report.  In the fast

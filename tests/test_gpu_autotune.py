@@ -41,12 +41,13 @@ def test_autotune_reports_and_keeps_results():
         g.init()
     rep = engine.autotune(b)
     chosen = rep["chosen"]
-    cands = {"part/2", "part/4", "part/8", "cr/128", "cr/256", "cr/512"}
+    cands = {"part/2", "part/4", "part/8", "cr/128", "cr/256", "cr/512", "seg/0"}
     assert len(chosen) == 4 and all(x in cands for x in chosen), rep
     # the same choices set by hand give the tuned run's bits
+    kinds = {"cr": engine.SOLVER_CR, "part": engine.SOLVER_PARTITION, "seg": engine.SOLVER_SEGMENTS}
     for (d, hyd), x in zip(((1, 1), (0, 1), (1, 0), (0, 0)), chosen):
         kind, param = x.split("/")
-        engine.set_solver(c, d, hyd, engine.SOLVER_CR if kind == "cr" else engine.SOLVER_PARTITION, int(param))
+        engine.set_solver(c, d, hyd, kinds[kind], int(param))
     for g in sims:
         g.step(6)
     for f in ("U", "V", "Z", "FS"):

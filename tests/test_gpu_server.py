@@ -118,3 +118,24 @@ def test_server_batches_runs_that_share_launches():
         for name in FIELDS:
             assert bits_equal(srv.result(rid, name), ref[name]), (rid, sc.name, name)
     srv.close()
+
+
+@pytest.mark.parametrize("nscen", [1, 3])
+def test_footprint_is_what_a_context_allocates(nscen):
+    """hd_footprint, which admission charges before anything is allocated,
+    equals the device bytes the context then holds (after init and a step,
+    with sediment), alone and batched."""
+    _case("footprint", nscen=nscen)
+    sc = S.meander_islands(72, 40, n_islands=3)
+    sc.sediment_start = sc.bed_change_start = 2.0
+    L = engine._server_lib()
+    if nscen == 1:
+        g = engine.Simulation(sc, form=FAST, sediment=True, bed_change=True, Tmax=8.0)
+    else:
+        g = engine.Batch([sc] * nscen, form=FAST, sediment=True, bed_change=True, Tmax=8.0)
+    g.init()
+    g.step(4)
+    held = engine.C.c_size_t()
+    engine.check(L.hd_memory_bytes(g.h, engine.C.byref(held)), g.h)
+    assert held.value == engine.footprint(sc, nscen=nscen)
+    g.close()

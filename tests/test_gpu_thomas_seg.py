@@ -4,7 +4,9 @@ field (the systems of a line are independent; the line walk carries
 values only across rows no system reads), on meshes with many systems
 per line, with one- and two-cell systems, drying fronts, every boundary
 type and the sediment sweeps.  The tuner picks between the two by time
-alone, so a tuned run equals an untuned one bit for bit."""
+alone, so a tuned run equals an untuned one bit for bit.  In the fast FP64
+form the same Thomas is one more solver choice, held to the oracle at the
+fast form's bars."""
 import numpy as np
 import pytest
 
@@ -73,11 +75,37 @@ def test_plain_tuning_changes_no_bit():
     b.close()
 
 
-def test_per_system_thomas_is_the_plain_forms():
-    sim = engine.Simulation(S.open_box(24, 18), form=abi.HD_FORM_FP64_FAST)
+def test_per_system_thomas_is_for_the_fp64_forms():
+    sim = engine.Simulation(S.open_box(24, 18), form=abi.HD_FORM_FP32_REFINE)
     with pytest.raises(engine.HydistError):
         engine.set_solver(sim, 0, 1, engine.SOLVER_SEGMENTS)
     sim.close()
+
+
+def test_thomas_per_system_in_the_fast_form():
+    """In the fast FP64 form the per-system Thomas is one more candidate
+    of the tuner: the same FP64 Thomas on the fast form's assembled rows
+    (the row sweeps then assemble and extract in kernels of their own),
+    within the fast form's bars of the oracle."""
+    sc = S.meander_islands(160, 96, n_islands=8)
+    sc.sediment_start = sc.bed_change_start = 4.0
+    print("[case] thomas per system, fast FP64", flush=True)
+    g, o = make_pair(sc, form=abi.HD_FORM_FP64_FAST, sediment=True, bed_change=True)
+    per_system(g)
+    g.step(10)
+    o.step(10)
+    assert np.array_equal(g.get("KHOUOT"), o.get("KHOUOT"))
+    for f in ("Z", "U", "V"):
+        d = float(np.max(np.abs(g.get(f) - o.get(f))))
+        print(f"[{f}] max abs error {d:.3e}", flush=True)
+        assert d <= 1e-13, (f, d)
+    a, b = g.get("FS"), o.get("FS")
+    e = float(np.max(np.abs(a - b))) / (np.abs(b).max() + 1e-300)
+    print(f"[FS] relative error {e:.3e}", flush=True)
+    assert e <= 1e-12
+    rep = engine.autotune(g)
+    assert any(k.endswith("/seg0") for k in rep), rep
+    g.close()
 
 
 def test_thomas_per_system_in_a_batch_with_graphs():
@@ -95,6 +123,32 @@ def test_thomas_per_system_in_a_batch_with_graphs():
     b.step(12)
     for k, sc in enumerate(scens):
         g = engine.Simulation(sc, sediment=True, bed_change=True)
+        g.init()
+        g.step(12)
+        for f in ["U", "V", "Z", "FS", "DH", "KHOUOT"]:
+            assert bits_equal(b.get(k, f), g.get(f).astype(np.float64)), (k, f)
+        g.close()
+    b.close()
+
+
+def test_fast_form_thomas_per_system_in_a_batch_with_graphs():
+    """The fast FP64 form's per-system Thomas in a batch under whole-step
+    graphs: every member equals its single run with the same solver, bit
+    for bit (a scenario's arithmetic does not depend on its batch)."""
+    scens = []
+    for k in range(3):
+        s = S.ensemble_member(k, M=96, N=48)
+        s.sediment_start = s.bed_change_start = 10.0
+        scens.append(s)
+    fast = abi.HD_FORM_FP64_FAST
+    print("[case] thomas per system, fast FP64, batch of 3 with graphs", flush=True)
+    b = engine.Batch(scens, form=fast, sediment=True, bed_change=True, graph=True)
+    per_system(b)
+    b.init()
+    b.step(12)
+    for k, sc in enumerate(scens):
+        g = engine.Simulation(sc, form=fast, sediment=True, bed_change=True)
+        per_system(g)
         g.init()
         g.step(12)
         for f in ["U", "V", "Z", "FS", "DH", "KHOUOT"]:
