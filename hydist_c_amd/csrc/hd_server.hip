@@ -71,7 +71,7 @@ struct Run {
     bool cancel = false;
     int rc = 0;
     std::string err;
-    Clock::time_point submitted, started, finished;
+    Clock::time_point submitted;
     std::vector<double> results[kNumResults];
 };
 
@@ -207,18 +207,30 @@ void hd_server::admit() {
 
 // Ends a group: copies its runs' results out when it completed, frees its
 // context and its budget, and records each run's end.  Called without mu
-// held, by the worker or the helper thread; the group is deleted.
+// held, by the worker or the helper thread; the group is deleted.  The
+// copies wait for the device, so they go into the group's own buffers
+// without mu and reach the runs (which API calls read under mu) with it.
 void hd_server::retire(Group *g, int state) {
+    const size_t nm = g->members.size();
+    std::vector<char> skip(nm);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (size_t m = 0; m < nm; m++) skip[m] = g->members[m]->cancel;
+    }
+    std::vector<std::vector<double>> out(nm * kNumResults);
+    std::vector<int> rcs(nm, 0);
+    std::vector<std::string> errs(nm);
     if (g->ctx && state == HD_RUN_DONE) {
         const size_t n = (size_t)(g->members[0]->mesh.M + 3) * (g->members[0]->mesh.N + 3);
-        for (Run *r : g->members) {
-            if (r->cancel) continue;
-            for (int k = 0; k < kNumResults && !r->rc; k++) {
-                r->results[k].resize(n);
-                int rc = hd_batch_copy_out(g->ctx, r->scen, kResultFields[k], r->results[k].data(), n);
+        for (size_t m = 0; m < nm; m++) {
+            if (skip[m]) continue;
+            for (int k = 0; k < kNumResults && !rcs[m]; k++) {
+                std::vector<double> &dst = out[m * kNumResults + k];
+                dst.resize(n);
+                int rc = hd_batch_copy_out(g->ctx, g->members[m]->scen, kResultFields[k], dst.data(), n);
                 if (rc) {
-                    r->rc = rc;
-                    r->err = hd_last_error(g->ctx);
+                    rcs[m] = rc;
+                    errs[m] = hd_last_error(g->ctx);
                 }
             }
         }
@@ -227,11 +239,15 @@ void hd_server::retire(Group *g, int state) {
     g->ctx = nullptr;
     std::lock_guard<std::mutex> lk(mu);
     const Clock::time_point now = Clock::now();
-    for (Run *r : g->members) {
+    for (size_t m = 0; m < nm; m++) {
+        Run *r = g->members[m];
         r->t = g->t;
         r->steps = g->steps;
-        r->finished = now;
-        if (g->rc && !r->rc) {
+        for (int k = 0; k < kNumResults; k++) r->results[k] = std::move(out[m * kNumResults + k]);
+        if (rcs[m]) {
+            r->rc = rcs[m];
+            r->err = errs[m];
+        } else if (g->rc) {
             r->rc = g->rc;
             r->err = g->err;
         }
@@ -273,12 +289,8 @@ void hd_server::create_loop() {
             g = to_create.front();
             to_create.pop_front();
         }
-        const Clock::time_point now = Clock::now();
         std::vector<hd_mesh> meshes;
-        for (Run *r : g->members) {
-            r->started = now;
-            meshes.push_back(r->mesh);
-        }
+        for (Run *r : g->members) meshes.push_back(r->mesh);
         const Run *r0 = g->members[0];
         int rc = hd_batch_create(meshes.data(), (int)meshes.size(), &r0->coeffs, &r0->ops, &r0->cfg, &g->ctx);
         if (!rc) rc = hd_set_graph(g->ctx, cfg.use_graph);

@@ -24,6 +24,10 @@ LIB = os.path.join(OUT, "libhydist_emu.so")
 # emulated device buffers are mmap'd and stay checked by their guard pages
 OUT_ASAN = os.path.join(ROOT, "tests", "_build", "emu_asan")
 ASAN = ["-fsanitize=address", "-fno-omit-frame-pointer"]
+# and with ThreadSanitizer (the serving loop's worker, its helper thread
+# and the API calls of client threads; the emulator announces its fibers)
+OUT_TSAN = os.path.join(ROOT, "tests", "_build", "emu_tsan")
+TSAN = ["-fsanitize=thread", "-fno-omit-frame-pointer"]
 LAUNCH = re.compile(r"([A-Za-z_][\w:]*(?:<[^<>;(){}]*>)?)\s*<<<(.*?)>>>\s*\(", re.S)
 
 
@@ -45,10 +49,14 @@ def _compile(args, extra=()):
 
 def build(verbose=False, sanitize=False):
     """Builds the emulated library and the compat driver against it into
-    tests/_build/emu (sanitize: with AddressSanitizer, into
-    tests/_build/emu_asan).  Returns the library's path."""
-    out = OUT_ASAN if sanitize else OUT
-    extra = ASAN if sanitize else []
+    tests/_build/emu (sanitize True or "address": with AddressSanitizer,
+    into tests/_build/emu_asan; "thread": with ThreadSanitizer, into
+    tests/_build/emu_tsan).  Returns the library's path."""
+    if sanitize not in (False, True, "address", "thread"):
+        raise ValueError(f"sanitize: {sanitize!r}")
+    tsan = sanitize == "thread"
+    out = OUT_TSAN if tsan else (OUT_ASAN if sanitize else OUT)
+    extra = TSAN if tsan else (ASAN if sanitize else [])
     lib = os.path.join(out, "libhydist_emu.so")
     src_dir = os.path.join(out, "src")
     os.makedirs(src_dir, exist_ok=True)

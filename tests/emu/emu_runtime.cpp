@@ -8,7 +8,11 @@
 // yield: a pass resumes every live lane once, so no lane leaves a barrier
 // before every lane has reached it.  Launches and allocations from several
 // host threads (the serving loop's worker and its context-making helper)
-// take one lock, so they run one after another, as on one queue.
+// take one lock, so they run one after another, as on one queue.  Graph
+// capture is per stream, as in HIP's relaxed capture mode: while one
+// thread captures its stream, launches on other streams run at once.
+// Built with -fsanitize=thread, the fibers are announced to
+// ThreadSanitizer, which then orders a workgroup's lanes by their switches.
 #include <signal.h>
 #include <sys/mman.h>
 #include <ucontext.h>
@@ -22,6 +26,13 @@
 #include <vector>
 
 #include "hip/hip_runtime.h"
+
+#if defined(__SANITIZE_THREAD__)
+#include <sanitizer/tsan_interface.h>
+#define EMU_TSAN 1
+#else
+#define EMU_TSAN 0
+#endif
 
 namespace {
 
@@ -54,22 +65,40 @@ struct Fiber {
     hd_emu::Lane lane;
     bool done;
     char *stack;
+    void *tsan;  // the fiber's ThreadSanitizer identity (sanitized builds)
 };
 constexpr size_t kStack = 256 * 1024;
 std::vector<Fiber> g_fibers;
 ucontext_t g_sched;
+void *g_sched_tsan = nullptr;  // the launching thread's ThreadSanitizer identity
 int g_cur = -1;
 const std::function<void()> *g_body = nullptr;
 void *g_dyn = nullptr;
 unsigned long long g_ballot[2][16];
 int g_ballot_parity[1024];
 
-void fiber_entry() {
+// Announces the switch to a fiber (or back to the launching thread) to
+// ThreadSanitizer; flag 0 orders what ran before the switch before what
+// runs after it.  Always inlined, so it adds no frame of its own to pop
+// after the switch.
+inline __attribute__((always_inline)) void tsan_switch(void *to) {
+#if EMU_TSAN
+    __tsan_switch_to_fiber(to, 0);
+#else
+    (void)to;
+#endif
+}
+
+// Not instrumented: its return, after the switch back to the launching
+// thread, must not pop that thread's ThreadSanitizer frames.
+__attribute__((no_sanitize_thread)) void fiber_entry() {
     (*g_body)();
     g_fibers[g_cur].done = true;
+    tsan_switch(g_sched_tsan);  // returns to g_sched through uc_link
 }
 
 bool g_capturing = false;
+hipStream_t g_capture_stream = nullptr;
 struct Node {
     const char *name;
     dim3 g, b;
@@ -97,10 +126,17 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
     if (g_fibers.size() < nt) {
         size_t old = g_fibers.size();
         g_fibers.resize(nt);
-        for (size_t i = old; i < nt; i++)
+        for (size_t i = old; i < nt; i++) {
             g_fibers[i].stack = (char *)mmap(nullptr, kStack, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS,
                                              -1, 0);
+#if EMU_TSAN
+            g_fibers[i].tsan = __tsan_create_fiber(0);
+#endif
+        }
     }
+#if EMU_TSAN
+    g_sched_tsan = __tsan_get_current_fiber();
+#endif
     g_body = &body;
     for (unsigned bz = 0; bz < g.z; bz++)
         for (unsigned by = 0; by < g.y; by++)
@@ -126,6 +162,7 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
                     for (unsigned t = 0; t < nt; t++) {
                         if (g_fibers[t].done) continue;
                         g_cur = (int)t;
+                        tsan_switch(g_fibers[t].tsan);
                         swapcontext(&g_sched, &g_fibers[t].ctx);
                         if (!g_fibers[t].done) alive = true;
                     }
@@ -180,7 +217,10 @@ struct Init {
 namespace hd_emu {
 dim3 g_block_idx, g_block_dim, g_grid_dim;
 Lane *cur() { return &g_fibers[g_cur].lane; }
-void sync() { swapcontext(&g_fibers[g_cur].ctx, &g_sched); }
+void sync() {
+    tsan_switch(g_sched_tsan);
+    swapcontext(&g_fibers[g_cur].ctx, &g_sched);
+}
 unsigned long long ballot(bool p) {
     const int t = g_cur, w = t / 64, lane = t % 64;
     int &par = g_ballot_parity[t];
@@ -194,8 +234,9 @@ unsigned long long ballot(bool p) {
     return r;
 }
 void *dyn_lds() { return g_dyn; }
-void launch(const char *name, dim3 g, dim3 b, size_t sm, hipStream_t, std::function<void()> body) {
-    if (g_capturing) {
+void launch(const char *name, dim3 g, dim3 b, size_t sm, hipStream_t st, std::function<void()> body) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (g_capturing && st == g_capture_stream) {
         g_capture->push_back(Node{name, g, b, sm, std::move(body)});
         return;
     }
@@ -255,21 +296,23 @@ extern "C" void *hd_emu_alloc(size_t n) {
 }
 extern "C" void hd_emu_free(void *p) { (void)hipFree(p); }
 
-static void no_capture(const char *what) {
-    if (g_capturing) {
+// A call HIP would refuse (or record wrongly) on a capturing stream.
+static void no_capture(const char *what, hipStream_t st) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (g_capturing && st == g_capture_stream) {
         std::fprintf(stderr, "[emu] %s during graph capture\n", what);
         std::abort();
     }
 }
 hipError_t hipMemset(void *p, int v, size_t n) { std::memset(p, v, n); return hipSuccess; }
-hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) {
-    no_capture("hipMemsetAsync");
+hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t st) {
+    no_capture("hipMemsetAsync", st);
     std::memset(p, v, n);
     return hipSuccess;
 }
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
-hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
-    no_capture("hipMemcpyAsync");
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t st) {
+    no_capture("hipMemcpyAsync", st);
     std::memmove(d, s, n);
     return hipSuccess;
 }
@@ -278,8 +321,8 @@ hipError_t hipMemcpy2D(void *d, size_t dp, const void *s, size_t sp, size_t w, s
     return hipSuccess;
 }
 hipError_t hipMemcpy2DAsync(void *d, size_t dp, const void *s, size_t sp, size_t w, size_t h, hipMemcpyKind k,
-                            hipStream_t) {
-    no_capture("hipMemcpy2DAsync");
+                            hipStream_t st) {
+    no_capture("hipMemcpy2DAsync", st);
     return hipMemcpy2D(d, dp, s, sp, w, h, k);
 }
 struct emu_stream {
@@ -291,7 +334,10 @@ struct emu_event {
 hipError_t hipStreamCreate(hipStream_t *s) { *s = new emu_stream{0}; return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
 hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
-hipError_t hipStreamSynchronize(hipStream_t) { no_capture("hipStreamSynchronize"); return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t st) {
+    no_capture("hipStreamSynchronize", st);
+    return hipSuccess;
+}
 hipError_t hipDeviceSynchronize() { return hipSuccess; }
 hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu_event{0}; return hipSuccess; }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
@@ -309,6 +355,7 @@ hipError_t hipFuncSetAttribute(const void *, hipFuncAttribute, int v) {
     return v <= 160 * 1024 ? hipSuccess : hipErrorInvalidValue;
 }
 hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *p) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     std::memset(a, 0, sizeof(*a));
     for (auto &kv : g_allocs)
         if ((uintptr_t)p >= kv.first && (uintptr_t)p < kv.first + kv.second.n) {
@@ -320,15 +367,19 @@ hipError_t hipPointerGetAttributes(hipPointerAttribute_t *a, const void *p) {
     a->hostPointer = (void *)p;
     return hipSuccess;
 }
-hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
-    if (g_capturing) return hipErrorInvalidValue;
+hipError_t hipStreamBeginCapture(hipStream_t st, hipStreamCaptureMode) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (g_capturing) return hipErrorInvalidValue;  // one capture at a time here
     g_capturing = true;
+    g_capture_stream = st;
     g_capture = new std::vector<Node>();
     return hipSuccess;
 }
-hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
-    if (!g_capturing) return hipErrorInvalidValue;
+hipError_t hipStreamEndCapture(hipStream_t st, hipGraph_t *g) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!g_capturing || st != g_capture_stream) return hipErrorInvalidValue;
     g_capturing = false;
+    g_capture_stream = nullptr;
     *g = new emu_graph{std::move(*g_capture)};
     delete g_capture;
     g_capture = nullptr;
@@ -339,6 +390,7 @@ hipError_t hipGraphInstantiate(hipGraphExec_t *x, hipGraph_t g, hipGraphNode_t *
     return hipSuccess;
 }
 hipError_t hipGraphLaunch(hipGraphExec_t x, hipStream_t) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
     for (auto &nd : x->nodes) run_launch(nd.name, nd.g, nd.b, nd.sm, nd.body);
     return hipSuccess;
 }
