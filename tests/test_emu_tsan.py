@@ -1,4 +1,4 @@
-"""The serving loop's threads under ThreadSanitizer, on the CPU: the
+"""The library's threads under ThreadSanitizer, on the CPU: the
 emulated build (tests/emu) compiled with -fsanitize=thread runs a server
 with whole-step graphs while three client threads submit, poll, read
 errors, cancel (queued and running runs), wait, fetch results and release,
@@ -68,6 +68,39 @@ print("server ok", st, flush=True)
 """
 
 
+CONTEXTS = r"""
+import sys, threading
+sys.path.insert(0, {root!r})
+from hydist_c_amd import abi, engine, scenarios as S
+engine.LIB_PATH = {lib!r}
+engine._lib = None
+errors = []
+
+def own(k, form):
+    try:
+        sc = S.open_box(16, 12, seed=40 + k)
+        sc.sediment_start = sc.bed_change_start = 2.0
+        g = engine.Simulation(sc, form=form, sediment=True, bed_change=True)
+        g.init()
+        if k == 0:
+            engine.autotune(g)
+        g.step(2)
+        g.get("Z")
+        g.close()
+    except Exception as e:
+        errors.append(repr(e))
+
+forms = [abi.HD_FORM_FP64_FAST, abi.HD_FORM_FP64_PLAIN, abi.HD_FORM_FP32_REFINE]
+ts = [threading.Thread(target=own, args=(k, f)) for k, f in enumerate(forms)]
+for t in ts:
+    t.start()
+for t in ts:
+    t.join()
+assert not errors, errors
+print("contexts ok", flush=True)
+"""
+
+
 @pytest.fixture(scope="module")
 def tsan_build():
     sys.path.insert(0, os.path.join(HERE, "emu"))
@@ -84,4 +117,16 @@ def test_serving_loop_under_tsan(tsan_build):
     r = subprocess.run([sys.executable, "-c", SERVER.format(root=ROOT, lib=lib)], capture_output=True, text=True,
                        timeout=900, env=env)
     assert r.returncode == 0 and "server ok" in r.stdout, r.stdout[-2000:] + r.stderr[-6000:]
+    assert "ThreadSanitizer" not in r.stderr, r.stderr[-6000:]
+
+
+def test_contexts_on_threads_under_tsan(tsan_build):
+    """Contexts of the three forms created, tuned, stepped and destroyed on
+    three threads at once (a context is used by one thread at a time;
+    different contexts share no host state)."""
+    lib, rt = tsan_build
+    env = dict(os.environ, LD_PRELOAD=rt, TSAN_OPTIONS="halt_on_error=1:exitcode=66")
+    r = subprocess.run([sys.executable, "-c", CONTEXTS.format(root=ROOT, lib=lib)], capture_output=True, text=True,
+                       timeout=900, env=env)
+    assert r.returncode == 0 and "contexts ok" in r.stdout, r.stdout[-2000:] + r.stderr[-6000:]
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-6000:]
