@@ -8,6 +8,7 @@ their bars, every line solver, the tridiagonal entry points, degenerate
 meshes, the serving loop with its worker and context-making threads)
 under the CPU suite; timing and occupancy stay GPU questions."""
 import os
+import re
 import subprocess
 import sys
 
@@ -18,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FILES = ["test_gpu_gtsv.py", "test_zz_no_alloc.py", "test_gpu_parity.py", "test_gpu_fast.py",
          "test_gpu_edge_meshes.py", "test_gpu_server.py", "test_gpu_determinism.py",
          "test_gpu_random_parity.py", "test_gpu_thomas_seg.py"]
-# the cases that finish in about two minutes on the emulator together
+# the cases that finish in about a minute on the emulator together (four
+# processes)
 SELECT = " or ".join([
     "gtsv", "noalloc", "test_server", "two_runs", "footprint_is", "(random_scenario and ([0- or [1- or [2-))",
     "(thomas_per_system and ([box] or [onecell] or [flats]))", "plain_tuning", "fp64_forms", "in_the_fast_form", "fast_form_thomas",
@@ -39,11 +41,13 @@ def emu_built():
 
 
 def test_gpu_suite_small_cases_on_the_emulator(emu_built):
-    env = dict(os.environ, HD_EMU="1")
-    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-k", SELECT] + \
+    env = dict(os.environ, HD_EMU="1", OMP_NUM_THREADS="2")  # the oracle's threads, per worker
+    # four worker processes (pytest-xdist): the cases are independent
+    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-n", "4", "-p", "no:cacheprovider", "-k", SELECT] + \
           [os.path.join(HERE, f) for f in FILES]
     r = subprocess.run(cmd, capture_output=True, text=True, env=env, cwd=os.path.dirname(HERE), timeout=900)
     tail = r.stdout[-3000:] + r.stderr[-2000:]
     assert r.returncode == 0, tail
-    assert " passed" in r.stdout and "deselected" in r.stdout, tail
+    m = re.search(r"(\d+) passed", r.stdout)
+    assert m and int(m.group(1)) >= 90, tail  # the selection above, not an empty one
     print(r.stdout.strip().splitlines()[-1])
