@@ -465,6 +465,8 @@ int hd_init(hd_ctx *ctx) {
 
 int hd_update_boundary(hd_ctx *ctx, double t) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    // the Q sides' weights hi come from hd_init (preprocess_data)
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
     LAUNCH(ctx, "k_boundary", k_boundary<<<ctx->nscen, 1024, 0, ctx->stream>>>(ctx->ds, t, 0));
     return HD_OK;
 }
@@ -795,6 +797,8 @@ int hd_get_segments(hd_ctx *ctx, int dir, int32_t *counts, size_t ncounts, int32
 
 int hd_mass_balance(hd_ctx *ctx, double *volume, double *inflow) {
     if (!ctx) return set_err(nullptr, HD_E_INVALID, "ctx is NULL");
+    // the wet mask and htaiz come from hd_init
+    if (!ctx->inited) return set_err(ctx, HD_E_STATE, "hd_init has not run");
     LAUNCH(ctx, "k_volume", k_volume<<<1, 1024, 0, ctx->stream>>>(ctx->ds));
     double acc[2];
     HD_HIP(ctx, hipMemcpyAsync(acc, ctx->hs.acc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));

@@ -150,7 +150,8 @@ int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *op
  * Vertical}, Htuongdoi and preprocess_data, on the device. */
 int hd_init(hd_ctx *ctx);
 
-/* update_boundary_at_t (engine.cpp:22-36) -> Update_Boundary_Value. */
+/* update_boundary_at_t (engine.cpp:22-36) -> Update_Boundary_Value.
+ * HD_E_STATE before hd_init, which derives the Q sides' weights. */
 int hd_update_boundary(hd_ctx *ctx, double t);
 
 /* First half-step (engine.cpp:66-170): t is the time after t += dT/2.
@@ -220,7 +221,8 @@ int hd_get_segments(hd_ctx *ctx, int dir, int32_t *counts, size_t ncounts,
 
 /* Domain water volume sum over wet compute cells of (htaiz + z)*dX*dY and
  * the running sum of boundary inflow volume, both accumulated in FP64 on
- * the device in a fixed order. */
+ * the device in a fixed order (either output may be NULL).  HD_E_STATE
+ * before hd_init. */
 int hd_mass_balance(hd_ctx *ctx, double *volume, double *inflow);
 
 /* NaN/Inf watchdog (the reference checks only for launch errors after

@@ -1,0 +1,104 @@
+"""The C ABI's argument and state checks: every entry point answers a NULL
+handle or buffer, a wrong size, an out-of-range field, side, direction or
+scenario, and a call out of order with its status code (HD_E_INVALID,
+HD_E_STATE), never a crash, and the context stays usable afterwards (the
+reference aborts on any failed CUDA call; a drop-in library returns)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hydist_c_amd import abi, engine, scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+INVALID, STATE = -1, -5
+
+
+def test_calls_out_of_order_and_bad_arguments():
+    print("[case] C ABI argument checks", flush=True)
+    L = engine.lib()
+    engine._server_lib()
+    sc = S.open_box(20, 14)
+    g = engine.Simulation(sc)
+    h = g.h
+    n = (sc.M + 3) * (sc.N + 3)
+    buf = np.zeros(n)
+    ib = np.zeros(n, dtype=np.int32)
+    dp = buf.ctypes.data_as(C.POINTER(C.c_double))
+    ip = ib.ctypes.data_as(C.POINTER(C.c_int32))
+    t = C.c_double(0.0)
+    # before hd_init: the time loop, its stages, the boundary update and
+    # the mass balance need the mask and weights init derives
+    assert L.hd_step(h, C.byref(t)) == STATE
+    assert L.hd_halfstep_uz(h, C.c_double(0.0)) == STATE
+    assert L.hd_halfstep_vz(h, C.c_double(0.0)) == STATE
+    assert L.hd_run(h, C.byref(t), C.c_double(4.0)) == STATE
+    assert L.hd_update_boundary(h, C.c_double(0.0)) == STATE
+    assert L.hd_mass_balance(h, C.byref(t), C.byref(t)) == STATE
+    assert "hd_init" in engine.lib().hd_last_error(h).decode()
+    # uploaded fields can be read before init
+    assert L.hd_copy_out(h, abi.F["Z"], dp, n) == 0
+    g.init()
+    # NULL handles and outputs
+    for fn in (L.hd_init, L.hd_sync):
+        assert fn(None) == INVALID
+    assert L.hd_destroy(None) == 0
+    assert L.hd_step(None, C.byref(t)) == INVALID
+    assert L.hd_step(h, None) == INVALID
+    assert L.hd_run(h, None, C.c_double(4.0)) == INVALID
+    assert L.hd_get_hmax(h, None) == INVALID
+    assert L.hd_set_hmax(h, None) == INVALID
+    assert L.hd_snapshot_size(h, None) == INVALID
+    assert L.hd_memory_bytes(h, None) == INVALID
+    assert L.hd_batch_size(h, None) == INVALID
+    assert L.hd_stage_stats(h, None, 10, 0) == INVALID
+    assert L.hd_device_info(None, 10) == INVALID
+    assert L.hd_device_info(C.create_string_buffer(4), 0) == INVALID
+    # field copies: buffer, size, field id and element type
+    assert L.hd_copy_out(h, abi.F["Z"], None, n) == INVALID
+    for m in (n - 1, n + 1, 0):
+        assert L.hd_copy_out(h, abi.F["Z"], dp, m) == INVALID
+        assert L.hd_copy_in(h, abi.F["Z"], dp, m) == INVALID
+    for f in (-1, max(abi.F.values()) + 1):
+        assert L.hd_copy_out(h, f, dp, n) == INVALID
+    assert L.hd_copy_out(h, abi.F["KHOUOT"], dp, n) == INVALID
+    assert L.hd_copy_out_i32(h, abi.F["Z"], ip, n) == INVALID
+    assert L.hd_copy_in(h, abi.F["Z"], None, n) == INVALID
+    assert L.hd_copy_in_i32(h, abi.F["KHOUOT"], None, n) == INVALID
+    # boundary arrays: side and size
+    for which, m in ((-1, 10), (5, 10), (0, 10 ** 6)):
+        assert L.hd_copy_out_boundary(h, which, dp, m) == INVALID
+    assert L.hd_copy_out_boundary(h, 0, None, 10) == INVALID
+    # segments, watchdog, fault injection, batch scenario
+    assert L.hd_get_segments(h, 2, None, 0, None, None, 0, None) == INVALID
+    assert L.hd_set_watchdog(h, -1) == INVALID
+    assert L.hd_inject_fault(h, 99) == INVALID
+    for k in (-1, 1):
+        assert L.hd_batch_copy_out(h, k, abi.F["Z"], dp, n) == INVALID
+    # snapshots: short buffers and a buffer that is no snapshot
+    sz = C.c_size_t()
+    assert L.hd_snapshot_size(h, C.byref(sz)) == 0
+    sb = C.create_string_buffer(sz.value)
+    assert L.hd_snapshot(h, sb, sz.value - 1, C.c_double(0.0)) == INVALID
+    assert L.hd_restore(h, sb, sz.value - 1, C.byref(t)) == INVALID
+    assert L.hd_restore(h, sb, sz.value, C.byref(t)) == INVALID
+    assert L.hd_save_checkpoint(h, None, C.c_double(0.0)) == INVALID
+    assert L.hd_save_checkpoint(h, b"/nonexistent-dir/x.ckpt", C.c_double(0.0)) == INVALID
+    assert L.hd_load_checkpoint(h, b"/nonexistent-dir/x.ckpt", C.byref(t)) == INVALID
+    # constructors and free functions
+    assert L.hd_create(None, None, None, None, None) == INVALID
+    assert L.hd_batch_create(None, 0, None, None, None, None) == INVALID
+    assert L.hd_footprint(None, None, 1, None) == INVALID
+    assert L.hd_gtsv_batch(None, None, None, None, None, 4, 1, 4, 0, None) == INVALID
+    assert L.hd_server_create(None, None) == INVALID
+    assert L.hd_server_poll(None, 1, None, None) == INVALID
+    # the context still steps, equal to a fresh one
+    g.step(3)
+    f = engine.Simulation(sc)
+    f.init()
+    f.step(3)
+    for name in ("U", "V", "Z"):
+        assert np.array_equal(g.get(name), f.get(name)), name
+    f.close()
+    g.close()
