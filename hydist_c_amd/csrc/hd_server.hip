@@ -490,7 +490,20 @@ int hd_server_cancel(hd_server *s, int64_t id) {
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->runs.find(id);
     if (it == s->runs.end()) return HD_E_INVALID;
-    it->second->cancel = true;
+    Run *r = it->second.get();
+    r->cancel = true;
+    // a run still in the queue (not yet in a group) leaves it at once, even
+    // while every slot is taken; one in a group ends at its group's next
+    // round boundary
+    if (r->state == HD_RUN_QUEUED && !r->group) {
+        auto q = std::find(s->queue.begin(), s->queue.end(), r);
+        if (q != s->queue.end()) {
+            s->queue.erase(q);
+            r->state = HD_RUN_CANCELLED;
+            s->cancelled++;
+            s->done.notify_all();
+        }
+    }
     s->wake.notify_all();
     return HD_OK;
 }

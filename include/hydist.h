@@ -420,6 +420,9 @@ int hd_server_wait(hd_server *s, int64_t id, double timeout_s, int *state);
  * (N+3) x (M+3). */
 int hd_server_result(hd_server *s, int64_t id, int field, double *dst, size_t n);
 int hd_server_error(hd_server *s, int64_t id, char *buf, size_t n);
+/* A queued run ends as cancelled at once (also while every slot is
+ * taken); a running one at its context's next round boundary, or, in a
+ * batched context, when that context ends. */
 int hd_server_cancel(hd_server *s, int64_t id);
 /* Drops a finished run's stored results. */
 int hd_server_release(hd_server *s, int64_t id);

@@ -102,3 +102,44 @@ def test_calls_out_of_order_and_bad_arguments():
         assert np.array_equal(g.get(name), f.get(name)), name
     f.close()
     g.close()
+
+
+def test_server_calls_with_bad_ids_sizes_and_states():
+    print("[case] server ABI checks", flush=True)
+    L = engine._server_lib()
+    sc = S.open_box(20, 14, seed=2)
+    srv = engine.Server(hbm_budget=engine.footprint(sc) + 1, max_active=1, steps_per_quantum=1, use_graph=False)
+    h = srv.h
+    st, tt = C.c_int(), C.c_double()
+    assert L.hd_server_poll(h, 12345, C.byref(st), C.byref(tt)) == INVALID
+    assert L.hd_server_cancel(h, 12345) == INVALID
+    assert L.hd_server_release(h, 12345) == INVALID
+    assert L.hd_server_stats(h, C.create_string_buffer(8), 8) == INVALID  # too small for the JSON
+    # a mesh below the smallest size is refused at submission
+    tiny = engine.MeshHolder(S.open_box(20, 14))
+    tiny.mesh.M = 3
+    rid = C.c_int64()
+    coeffs, ops = engine.load_coeffs(), engine.make_options(sc, 8.0)
+    cfg = engine.make_config()
+    assert L.hd_server_submit(h, C.byref(tiny.mesh), C.byref(coeffs), C.byref(ops), C.byref(cfg),
+                              C.byref(rid)) == INVALID
+    a = srv.submit(sc, 40000.0)  # holds the one slot until cancelled
+    b = srv.submit(S.open_box(20, 14, seed=3), 8.0)  # queued behind it
+    out = np.zeros((sc.N + 3, sc.M + 3))
+    dp = out.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.hd_server_release(h, b) == STATE  # not finished
+    assert L.hd_server_result(h, b, abi.F["Z"], dp, out.size) == STATE
+    assert srv.wait(b, timeout=0.0) < engine.RUN_DONE
+    srv.cancel(b)
+    assert srv.wait(b) == engine.RUN_CANCELLED
+    assert L.hd_server_result(h, b, abi.F["Z"], dp, out.size) == -8  # HD_E_CANCELLED
+    srv.cancel(a)
+    assert srv.wait(a) == engine.RUN_CANCELLED
+    a = srv.submit(sc, 8.0)
+    assert srv.wait(a) == engine.RUN_DONE
+    assert L.hd_server_result(h, a, abi.F["Z"], dp, out.size - 1) == INVALID
+    assert L.hd_server_result(h, a, abi.F["HMOI"], dp, out.size) == INVALID  # not a result field
+    assert L.hd_server_result(h, a, abi.F["Z"], dp, out.size) == 0
+    assert L.hd_server_release(h, a) == 0
+    assert L.hd_server_poll(h, a, C.byref(st), C.byref(tt)) == INVALID  # released
+    srv.close()
