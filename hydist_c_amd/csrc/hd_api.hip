@@ -946,11 +946,30 @@ int hd_memory_bytes(hd_ctx *ctx, size_t *bytes) {
 namespace {
 
 struct SnapHeader {
-    char magic[8];  // "HDSNAP2"
+    char magic[8];  // "HDSNAP3"
     int32_t M, N, P, H, capH, capV, nscen, nbuf;
     double t;
     uint64_t bytes;
+    uint64_t sum;  // snap_sum of the header (with sum = 0) and the payload
 };
+
+// FNV-1a over 8-byte words (then the tail bytes), continuing from h: a
+// flipped or truncated byte of a checkpoint file changes it.
+uint64_t fnv_words(const char *p, size_t n, uint64_t h) {
+    size_t k = 0;
+    for (; k + 8 <= n; k += 8) {
+        uint64_t w;
+        std::memcpy(&w, p + k, 8);
+        h = (h ^ w) * 0x100000001b3ull;
+    }
+    for (; k < n; k++) h = (h ^ (unsigned char)p[k]) * 0x100000001b3ull;
+    return h;
+}
+
+uint64_t snap_sum(SnapHeader h, const char *payload, size_t n) {
+    h.sum = 0;
+    return fnv_words(payload, n, fnv_words(reinterpret_cast<const char *>(&h), sizeof(h), 0xcbf29ce484222325ull));
+}
 
 // Every buffer whose contents carry over from one step to the next.
 std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
@@ -984,7 +1003,7 @@ std::vector<std::pair<void *, size_t>> snap_buffers(hd_ctx *ctx) {
 
 SnapHeader snap_header(hd_ctx *ctx, double t, size_t nbuf, size_t bytes) {
     SnapHeader h{};
-    std::memcpy(h.magic, "HDSNAP2", 8);
+    std::memcpy(h.magic, "HDSNAP3", 8);
     const DevState &s = ctx->hs;
     h.M = s.M; h.N = s.N; h.P = s.P; h.H = s.H; h.capH = s.capH; h.capV = s.capV;
     h.nscen = ctx->nscen;
@@ -1013,14 +1032,15 @@ int hd_snapshot(hd_ctx *ctx, void *buf, size_t n, double t) {
     if (n < need) return set_err(ctx, HD_E_INVALID, "snapshot buffer too small");
     auto bufs = snap_buffers(ctx);
     SnapHeader h = snap_header(ctx, t, bufs.size(), need);
-    char *p = static_cast<char *>(buf);
-    std::memcpy(p, &h, sizeof(h));
-    p += sizeof(h);
+    char *const base = static_cast<char *>(buf);
+    char *p = base + sizeof(h);
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto &b : bufs) {
         HD_HIP(ctx, hipMemcpy(p, b.first, b.second, hipMemcpyDeviceToHost));
         p += b.second;
     }
+    h.sum = snap_sum(h, base + sizeof(h), need - sizeof(h));
+    std::memcpy(base, &h, sizeof(h));
     return HD_OK;
 }
 
@@ -1039,6 +1059,8 @@ int hd_restore(hd_ctx *ctx, const void *buf, size_t n, double *t) {
         h.bytes != need || n < need)
         return set_err(ctx, HD_E_INVALID, "snapshot does not match this context's mesh and layout");
     const char *p = static_cast<const char *>(buf) + sizeof(h);
+    if (snap_sum(h, p, need - sizeof(h)) != h.sum)
+        return set_err(ctx, HD_E_INVALID, "snapshot checksum mismatch (damaged or truncated)");
     HD_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (auto &b : bufs) {
         HD_HIP(ctx, hipMemcpy(b.first, p, b.second, hipMemcpyHostToDevice));

@@ -370,7 +370,9 @@ int hd_gtsv_long_noalloc(const double *dl, const double *d, const double *du, co
  * fields, boundary arrays, segment tables, inflow accumulator) and the
  * time t, in the device layout; it restores into a context created from
  * the same mesh and config, and the run then continues bit for bit as if
- * never interrupted. */
+ * never interrupted.  The header keeps a checksum of itself and the payload:
+ * hd_restore refuses (HD_E_INVALID) a snapshot of another mesh or layout
+ * and a damaged or truncated one, leaving the context as it was. */
 int hd_snapshot_size(hd_ctx *ctx, size_t *bytes);
 int hd_snapshot(hd_ctx *ctx, void *buf, size_t n, double t);
 int hd_restore(hd_ctx *ctx, const void *buf, size_t n, double *t);

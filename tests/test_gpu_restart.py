@@ -44,3 +44,33 @@ def test_restore_rejects_other_mesh(tmp_path):
         b.load_checkpoint(ck)
     a.close()
     b.close()
+
+
+def test_restore_rejects_a_damaged_checkpoint(tmp_path):
+    """One flipped byte (in the header's time or anywhere in the payload),
+    or a truncated file, is refused, and the context keeps its state."""
+    print("[case] damaged checkpoint", flush=True)
+    sc = S.open_box(40, 28, seed=3)
+    a = engine.Simulation(sc)
+    a.init()
+    a.step(4)
+    ck = tmp_path / "a.hdck"
+    a.save_checkpoint(ck)
+    raw = ck.read_bytes()
+    b = engine.Simulation(sc)
+    b.init()
+    z0 = b.get("Z")
+    for pos in (40, 200, len(raw) // 2, len(raw) - 1):  # 40: the time t
+        bad = bytearray(raw)
+        bad[pos] ^= 0x10
+        (tmp_path / "bad.hdck").write_bytes(bytes(bad))
+        with pytest.raises(engine.HydistError, match="checksum"):
+            b.load_checkpoint(tmp_path / "bad.hdck")
+    (tmp_path / "short.hdck").write_bytes(raw[:-8])
+    with pytest.raises(engine.HydistError):
+        b.load_checkpoint(tmp_path / "short.hdck")
+    assert bits_equal(b.get("Z"), z0)
+    b.load_checkpoint(ck)
+    assert bits_equal(b.get("Z"), a.get("Z"))
+    a.close()
+    b.close()
