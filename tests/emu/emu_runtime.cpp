@@ -13,6 +13,11 @@
 // thread captures its stream, launches on other streams run at once.
 // Built with -fsanitize=thread, the fibers are announced to
 // ThreadSanitizer, which then orders a workgroup's lanes by their switches.
+// HD_EMU_ORDER=reverse runs the workgroups of a launch, and the lanes of
+// each pass between barriers, in reverse order: a result that changes
+// with it depends on an order the hardware does not promise (a missing
+// barrier, or one workgroup reading what another writes in the same
+// launch).
 #include <signal.h>
 #include <sys/mman.h>
 #include <ucontext.h>
@@ -99,6 +104,7 @@ __attribute__((no_sanitize_thread)) void fiber_entry() {
 
 bool g_capturing = false;
 hipStream_t g_capture_stream = nullptr;
+bool g_reverse = false;  // HD_EMU_ORDER=reverse
 struct Node {
     const char *name;
     dim3 g, b;
@@ -138,37 +144,39 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
     g_sched_tsan = __tsan_get_current_fiber();
 #endif
     g_body = &body;
-    for (unsigned bz = 0; bz < g.z; bz++)
-        for (unsigned by = 0; by < g.y; by++)
-            for (unsigned bx = 0; bx < g.x; bx++) {
-                hd_emu::g_block_idx = dim3(bx, by, bz);
-                if (sm) std::memset(g_dyn, 0xcd, sm);
-                for (unsigned t = 0; t < nt; t++) {
-                    Fiber &f = g_fibers[t];
-                    f.lane.tid = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
-                    f.lane.linear = (int)t;
-                    f.done = false;
-                    g_ballot_parity[t] = 0;
-                    getcontext(&f.ctx);
-                    f.ctx.uc_stack.ss_sp = f.stack;
-                    f.ctx.uc_stack.ss_size = kStack;
-                    f.ctx.uc_link = &g_sched;
-                    makecontext(&f.ctx, fiber_entry, 0);
-                }
-                std::memset(g_ballot, 0, sizeof(g_ballot));
-                bool alive = true;
-                while (alive) {
-                    alive = false;
-                    for (unsigned t = 0; t < nt; t++) {
-                        if (g_fibers[t].done) continue;
-                        g_cur = (int)t;
-                        tsan_switch(g_fibers[t].tsan);
-                        swapcontext(&g_sched, &g_fibers[t].ctx);
-                        if (!g_fibers[t].done) alive = true;
-                    }
-                }
-                g_cur = -1;
+    const size_t nblocks = (size_t)g.x * g.y * g.z;
+    for (size_t q = 0; q < nblocks; q++) {
+        const size_t blk = g_reverse ? nblocks - 1 - q : q;
+        hd_emu::g_block_idx = dim3((unsigned)(blk % g.x), (unsigned)(blk / g.x % g.y),
+                                   (unsigned)(blk / ((size_t)g.x * g.y)));
+        if (sm) std::memset(g_dyn, 0xcd, sm);
+        for (unsigned t = 0; t < nt; t++) {
+            Fiber &f = g_fibers[t];
+            f.lane.tid = dim3(t % b.x, (t / b.x) % b.y, t / (b.x * b.y));
+            f.lane.linear = (int)t;
+            f.done = false;
+            g_ballot_parity[t] = 0;
+            getcontext(&f.ctx);
+            f.ctx.uc_stack.ss_sp = f.stack;
+            f.ctx.uc_stack.ss_size = kStack;
+            f.ctx.uc_link = &g_sched;
+            makecontext(&f.ctx, fiber_entry, 0);
+        }
+        std::memset(g_ballot, 0, sizeof(g_ballot));
+        bool alive = true;
+        while (alive) {
+            alive = false;
+            for (unsigned k = 0; k < nt; k++) {
+                const unsigned t = g_reverse ? nt - 1 - k : k;
+                if (g_fibers[t].done) continue;
+                g_cur = (int)t;
+                tsan_switch(g_fibers[t].tsan);
+                swapcontext(&g_sched, &g_fibers[t].ctx);
+                if (!g_fibers[t].done) alive = true;
             }
+        }
+        g_cur = -1;
+    }
     if (sm) munmap(lds.base, lds.total);
     g_dyn = nullptr;
     g_kernel = "(host)";
@@ -209,6 +217,8 @@ struct Init {
         sa.sa_flags = SA_SIGINFO | SA_ONSTACK;
         sigaction(SIGSEGV, &sa, nullptr);
         sigaction(SIGBUS, &sa, nullptr);
+        const char *order = std::getenv("HD_EMU_ORDER");
+        g_reverse = order && std::string(order) == "reverse";
     }
 } g_init;
 

@@ -34,6 +34,15 @@ SELECT = " or ".join([
 ])
 
 
+# a cross-section of the kernels for the reversed-order run
+REVERSE = " or ".join([
+    "gtsv", "noalloc", "(random_scenario and ([0- or [1-))", "(thomas_per_system and [box])",
+    "(test_plain_fp64_bitwise and 12_0)", "sediment_and_bed_change", "(test_fast_form_close_to_oracle and 10_)",
+    "test_fast_form_sediment", "(test_every_line_solver_close_to_oracle and part4)",
+    "(edge_mesh_fast and (stripes or onecell))",
+])
+
+
 @pytest.fixture(scope="module")
 def emu_built():
     sys.path.insert(0, os.path.join(HERE, "emu"))
@@ -41,14 +50,28 @@ def emu_built():
     build_emu.build()
 
 
-def test_gpu_suite_small_cases_on_the_emulator(emu_built):
-    env = dict(os.environ, HD_EMU="1", OMP_NUM_THREADS="2")  # the oracle's threads, per worker
+def run_cases(select, minimum, **env_extra):
+    env = dict(os.environ, HD_EMU="1", OMP_NUM_THREADS="2", **env_extra)  # the oracle's threads, per worker
     # four worker processes (pytest-xdist): the cases are independent
-    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-n", "4", "-p", "no:cacheprovider", "-k", SELECT] + \
+    cmd = [sys.executable, "-m", "pytest", "-q", "-x", "-n", "4", "-p", "no:cacheprovider", "-k", select] + \
           [os.path.join(HERE, f) for f in FILES]
     r = subprocess.run(cmd, capture_output=True, text=True, env=env, cwd=os.path.dirname(HERE), timeout=900)
     tail = r.stdout[-3000:] + r.stderr[-2000:]
     assert r.returncode == 0, tail
     m = re.search(r"(\d+) passed", r.stdout)
-    assert m and int(m.group(1)) >= 90, tail  # the selection above, not an empty one
+    assert m and int(m.group(1)) >= minimum, tail  # the selection, not an empty one
     print(r.stdout.strip().splitlines()[-1])
+
+
+def test_gpu_suite_small_cases_on_the_emulator(emu_built):
+    run_cases(SELECT, 90)
+
+
+def test_results_do_not_depend_on_lane_or_workgroup_order(emu_built):
+    """The same checks with every launch's workgroups, and the lanes of each
+    pass between barriers, run in reverse order (HD_EMU_ORDER=reverse): a
+    kernel that reads LDS another lane writes without a barrier between, or
+    global memory another workgroup of the same launch writes, gives other
+    results and fails them (the plain form against the oracle bit for
+    bit)."""
+    run_cases(REVERSE, 20, HD_EMU_ORDER="reverse")
