@@ -6,6 +6,8 @@ Kernel launches `k<<<grid, block, lds, stream>>>(args)` are rewritten to
 `extern __shared__` array to the launch's dynamic LDS buffer.  Built with
 -ffp-contract=off like the HIP library and the oracle, so results can be
 compared bit for bit."""
+import fcntl
+import glob
 import os
 import re
 import subprocess
@@ -47,7 +49,36 @@ def _compile(args, extra=()):
     return obj
 
 
-def build(verbose=False, sanitize=False):
+def _inputs():
+    """Every file a build reads: the library's sources and headers, the
+    emulated runtime, the compat driver and this script."""
+    files = glob.glob(os.path.join(hb.CSRC, "*")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    files += glob.glob(os.path.join(HERE, "*.cpp")) + glob.glob(os.path.join(HERE, "hip", "*.h"))
+    files += [os.path.join(ROOT, "tests", "compat", "compat_driver.cpp"), os.path.abspath(__file__)]
+    return [f for f in files if os.path.isfile(f)]
+
+
+def build(verbose=False, sanitize=False, force=False):
+    """Builds unless the outputs are newer than every input (force: always);
+    concurrent callers (test workers) wait on a lock and build once."""
+    if sanitize not in (False, True, "address", "thread"):
+        raise ValueError(f"sanitize: {sanitize!r}")
+    tsan = sanitize == "thread"
+    out = OUT_TSAN if tsan else (OUT_ASAN if sanitize else OUT)
+    os.makedirs(out, exist_ok=True)
+    lib = os.path.join(out, "libhydist_emu.so")
+    drv = os.path.join(out, "compat_driver")
+    with open(os.path.join(out, ".lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        newest = max(os.path.getmtime(f) for f in _inputs())
+        if not force and all(os.path.exists(f) and os.path.getmtime(f) > newest for f in (lib, drv)):
+            if verbose:
+                print("up to date", lib, drv)
+            return lib
+        return _build(verbose, sanitize)
+
+
+def _build(verbose=False, sanitize=False):
     """Builds the emulated library and the compat driver against it into
     tests/_build/emu (sanitize True or "address": with AddressSanitizer,
     into tests/_build/emu_asan; "thread": with ThreadSanitizer, into
@@ -90,4 +121,4 @@ def build(verbose=False, sanitize=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--force" in sys.argv)
