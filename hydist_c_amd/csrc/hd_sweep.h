@@ -22,7 +22,7 @@ struct SweepBufs {
     double *A, *B, *C, *D, *X, *Ap, *Bp;
     signed char *code;
     int *sys0, *sysn;
-    int *segl;    // plain form, one lane per system: the sweep's systems (index into sys0 / sysn)
+    int *segl;    // Thomas with one lane per system: the sweep's systems (index into sys0 / sysn)
     int *segcnt;  // and their number, per scenario
     int ld;          // leading dimension of the active layout
     int ld_rm[2];    // [dir]: lines per row in ROW_MAJOR
