@@ -1626,8 +1626,8 @@ static bool solver_extracts(const hd_ctx *ctx, bool hyd) {
     return kind == HD_SOLVER_DEFAULT || kind == HD_SOLVER_PARTITION;
 }
 
+// ---------------------------------------------------------------------------
 // Thomas with one lane per system (HD_SOLVER_SEGMENTS, the FP64 forms).  The
-// Thomas with one lane per system (plain form, HD_SOLVER_SEGMENTS).  The
 // systems of a line are independent (identity rows between them), and
 // k_thomas's walk along a line carries values only across rows outside
 // every system, which no system reads: its first row divides by its own
