@@ -399,41 +399,33 @@ def ensemble_member(k, M=512, N=256):
     return s
 
 
-def segment_histogram(h, M, N):
-    """Wet-segment lengths along rows and columns (reference segmentation
-    rules) from a corner-depth field, for generator reports."""
+def segment_lengths(h, M, N):
+    """Wet-segment lengths of a corner-depth field under the reference's
+    segmentation rules (Find_Calculation_limits_Horizontal/Vertical,
+    support_funcs.cpp:238-343: a line is scanned from 2 to its end, a run
+    starting at end - 1 or later is dropped, a run wet through the end
+    includes it), for generator reports: {"rows": lengths along j of the
+    lines i = 2..N, "cols": lengths along i of the lines j = 2..M}."""
     wet = _wet_mask(h)
-    kh = np.where(wet, 0, 2)
-    kh[0, :] = 2
-    kh[:, 0] = 2
-    lens = []
-    for line, end in ((kh[i, :], M) for i in range(2, N + 1)):
-        lens.extend(_runs(line, end))
-    for line, end in ((kh[:, j], N) for j in range(2, M + 1)):
-        lens.extend(_runs(line, end))
-    return np.asarray(lens)
-
-
-def _runs(line, end):
-    out = []
-    start = 2
-    while start < end:
-        if line[start] != 0:
-            while line[start] and start < end:
-                start += 1
-        if start + 1 == end:
-            start = end
-        if line[start] == 0 and start + 1 < end:
-            e = start
-            while line[e] == 0 and e < end:
-                e += 1
-            if line[e] != 0 and e <= end:
-                out.append(e - 1 - start + 1)
-                start = e
-            else:
-                out.append(end - start + 1)
-                start = end
+    w = wet[2:N + 1, 2:M + 1]  # positions 2..end of every line
+    out = {}
+    for key, lines, end in (("rows", w, M), ("cols", w.T, N)):
+        pad = np.zeros((lines.shape[0], lines.shape[1] + 2), dtype=np.int8)
+        pad[:, 1:-1] = lines
+        d = np.diff(pad, axis=1)
+        s0 = np.nonzero(d == 1)[1]   # run starts, as position - 2
+        e0 = np.nonzero(d == -1)[1]  # one past run ends, as position - 1
+        start, last = s0 + 2, e0 + 1
+        keep = start <= end - 2
+        out[key] = (last - start + 1)[keep]
     return out
+
+
+def segment_histogram(h, M, N, edges=(1, 3, 10, 100, 1000, 10000)):
+    """Segment counts per length bin [edges[k], edges[k+1]) for rows and
+    columns (segment_lengths)."""
+    lens = segment_lengths(h, M, N)
+    return {k: np.histogram(v, bins=edges)[0] for k, v in lens.items()}
 
 
 def tidal_flats(M, N, seed=6, shelf=(0.03, 0.16), channel_depth=3.0, drop=2.0, rise_after=True,
