@@ -14,17 +14,21 @@
 // Built with -fsanitize=thread, the fibers are announced to
 // ThreadSanitizer, which then orders a workgroup's lanes by their switches.
 // HD_EMU_ORDER=reverse runs the workgroups of a launch, and the lanes of
-// each pass between barriers, in reverse order: a result that changes
-// with it depends on an order the hardware does not promise (a missing
-// barrier, or one workgroup reading what another writes in the same
-// launch).
+// each pass between barriers, in reverse order; HD_EMU_ORDER=shuffle:SEED
+// in a pseudo-random order drawn anew for every launch and pass.  A result
+// that changes with either depends on an order the hardware does not
+// promise (a missing barrier, or one workgroup reading what another writes
+// in the same launch).
 #include <signal.h>
 #include <sys/mman.h>
 #include <ucontext.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <numeric>
+#include <random>
 #include <map>
 #include <mutex>
 #include <string>
@@ -105,6 +109,8 @@ __attribute__((no_sanitize_thread)) void fiber_entry() {
 bool g_capturing = false;
 hipStream_t g_capture_stream = nullptr;
 bool g_reverse = false;  // HD_EMU_ORDER=reverse
+bool g_shuffle = false;  // HD_EMU_ORDER=shuffle:SEED
+std::mt19937_64 g_rng;
 struct Node {
     const char *name;
     dim3 g, b;
@@ -145,8 +151,17 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
 #endif
     g_body = &body;
     const size_t nblocks = (size_t)g.x * g.y * g.z;
+    std::vector<size_t> border;
+    std::vector<unsigned> lorder(nt);
+    std::iota(lorder.begin(), lorder.end(), 0u);
+    if (g_reverse) std::reverse(lorder.begin(), lorder.end());
+    if (g_shuffle) {
+        border.resize(nblocks);
+        std::iota(border.begin(), border.end(), (size_t)0);
+        std::shuffle(border.begin(), border.end(), g_rng);
+    }
     for (size_t q = 0; q < nblocks; q++) {
-        const size_t blk = g_reverse ? nblocks - 1 - q : q;
+        const size_t blk = g_shuffle ? border[q] : (g_reverse ? nblocks - 1 - q : q);
         hd_emu::g_block_idx = dim3((unsigned)(blk % g.x), (unsigned)(blk / g.x % g.y),
                                    (unsigned)(blk / ((size_t)g.x * g.y)));
         if (sm) std::memset(g_dyn, 0xcd, sm);
@@ -166,8 +181,9 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
         bool alive = true;
         while (alive) {
             alive = false;
+            if (g_shuffle) std::shuffle(lorder.begin(), lorder.end(), g_rng);
             for (unsigned k = 0; k < nt; k++) {
-                const unsigned t = g_reverse ? nt - 1 - k : k;
+                const unsigned t = lorder[k];
                 if (g_fibers[t].done) continue;
                 g_cur = (int)t;
                 tsan_switch(g_fibers[t].tsan);
@@ -219,6 +235,10 @@ struct Init {
         sigaction(SIGBUS, &sa, nullptr);
         const char *order = std::getenv("HD_EMU_ORDER");
         g_reverse = order && std::string(order) == "reverse";
+        if (order && std::string(order).rfind("shuffle:", 0) == 0) {
+            g_shuffle = true;
+            g_rng.seed(std::strtoull(order + 8, nullptr, 10));
+        }
     }
 } g_init;
 

@@ -1,7 +1,9 @@
 """Every form gives the same bits whatever order the workgroups of a launch
 and the lanes between barriers run in: the library's kernels, run on the
-CPU emulator (tests/emu) once in launch order and once in reverse order
-(HD_EMU_ORDER=reverse), in child processes, end with identical fields.
+CPU emulator (tests/emu) in launch order, in reverse order
+(HD_EMU_ORDER=reverse) and in a pseudo-random order drawn anew for every
+launch and pass between barriers (HD_EMU_ORDER=shuffle:SEED), in child
+processes, end with identical fields.
 The only atomics are bit-ors and the reductions run in a fixed order, so
 any difference is a race (a missing barrier, or a workgroup reading what
 another writes in the same launch), which on the GPU would make results
@@ -67,7 +69,7 @@ def emu_built():
 
 def run(tmp_path, case, order):
     form, solver, param = CASES[case]
-    path = str(tmp_path / f"{case}_{order}.npz")
+    path = str(tmp_path / f"{case}_{order.replace(':', '_')}.npz")
     code = RUN.format(root=ROOT, tests=HERE, form=form, solver=solver, param=param, path=path)
     env = dict(os.environ, HD_EMU_ORDER=order, OMP_NUM_THREADS="1")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
@@ -76,13 +78,16 @@ def run(tmp_path, case, order):
         return {k: z[k] for k in z.files}
 
 
-def test_same_bits_in_reverse_order(emu_built, tmp_path):
-    jobs = [(c, o) for c in CASES for o in ("forward", "reverse")]
+def test_same_bits_in_any_order(emu_built, tmp_path):
+    orders = ("forward", "reverse", "shuffle:7")
+    jobs = [(c, o) for c in CASES for o in orders]
     with ThreadPoolExecutor(max_workers=4) as ex:  # four child processes at a time
         res = dict(zip(jobs, ex.map(lambda j: run(tmp_path, *j), jobs)))
     for case in CASES:
-        a, b = res[(case, "forward")], res[(case, "reverse")]
-        assert a.keys() == b.keys()
-        for k in a:
-            assert np.array_equal(a[k].view(np.uint64) if a[k].dtype == np.float64 else a[k],
-                                  b[k].view(np.uint64) if b[k].dtype == np.float64 else b[k]), (case, k)
+        a = res[(case, "forward")]
+        for order in orders[1:]:
+            b = res[(case, order)]
+            assert a.keys() == b.keys()
+            for k in a:
+                assert np.array_equal(a[k].view(np.uint64) if a[k].dtype == np.float64 else a[k],
+                                      b[k].view(np.uint64) if b[k].dtype == np.float64 else b[k]), (case, order, k)
