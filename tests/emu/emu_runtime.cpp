@@ -126,6 +126,10 @@ void run_launch(const char *name, dim3 g, dim3 b, size_t sm, const std::function
         std::fprintf(stderr, "[emu] %s: bad block size %u\n", name, nt);
         std::abort();
     }
+    if (g.x == 0 || g.y == 0 || g.z == 0 || g.x > 0x7fffffffu || g.y > 65535 || g.z > 65535) {
+        std::fprintf(stderr, "[emu] %s: grid (%u, %u, %u) outside the device's limits\n", name, g.x, g.y, g.z);
+        std::abort();
+    }
     if (sm > 160 * 1024) {
         std::fprintf(stderr, "[emu] %s: dynamic LDS %zu > 160 KiB\n", name, sm);
         std::abort();
