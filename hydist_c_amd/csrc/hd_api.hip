@@ -307,6 +307,18 @@ int run_sweep(hd_ctx *ctx, const DevState *S, const DevState &s, int n, int dir,
     return run_line_sweep(ctx, S, s, n, dir, true, start, end, 0.0, 0.0);
 }
 
+int check_setup(const hd_coeffs *coeffs, const hd_config *cfg) {
+    const int form = cfg ? cfg->form : HD_FORM_FP64_PLAIN;
+    if (form < HD_FORM_FP64_PLAIN || form > HD_FORM_FP32_REFINE)
+        return set_err(nullptr, HD_E_INVALID, "unknown form");
+    if (cfg && (cfg->quirks & ~HD_QUIRK_Q11_LITERAL))
+        return set_err(nullptr, HD_E_INVALID, "unknown quirk bits (only HD_QUIRK_Q11_LITERAL is defined)");
+    for (double v : {coeffs->dX, coeffs->dY, coeffs->dT})
+        if (!(v > 0.0) || !std::isfinite(v))  // NaN fails v > 0
+            return set_err(nullptr, HD_E_INVALID, "dX, dY and dT must be positive and finite");
+    return HD_OK;
+}
+
 }  // namespace hd
 
 // ===========================================================================
@@ -406,9 +418,9 @@ static int create_ctx(const hd_mesh *meshes, int n, const hd_coeffs *coeffs, con
             meshes[k].total_time != meshes[0].total_time)
             return set_err(nullptr, HD_E_INVALID, "batched scenarios need one M, N and total_time");
     }
+    int rc0 = hd::check_setup(coeffs, cfg);
+    if (rc0) return rc0;
     const int form = cfg ? cfg->form : HD_FORM_FP64_PLAIN;
-    if (form < HD_FORM_FP64_PLAIN || form > HD_FORM_FP32_REFINE)
-        return set_err(nullptr, HD_E_INVALID, "unknown form");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return set_err(nullptr, HD_E_NODEV, "no HIP device");

@@ -97,6 +97,9 @@ int prof_resolve(hd_ctx *ctx);
 int alloc_scenario(hd_ctx *ctx, const hd_mesh *mesh, DevState &s);
 int alloc_shared(hd_ctx *ctx, int nscen);
 size_t footprint(const hd_mesh *mesh, int dense, int nscen);
+// HD_OK, or HD_E_INVALID (message set) for an unknown form or quirk bit, or
+// a grid spacing or time step that is not a positive finite number.
+int check_setup(const hd_coeffs *coeffs, const hd_config *cfg);
 
 int run_segmentation(hd_ctx *ctx, const DevState *S, const DevState &s, int n);
 int run_init(hd_ctx *ctx, const DevState *S, const DevState &s, int n);

@@ -398,6 +398,7 @@ int hd_server_submit(hd_server *s, const hd_mesh *mesh, const hd_coeffs *coeffs,
         return HD_E_INVALID;
     for (int k = 0; k < 4; k++)
         if (!mesh->bc[k]) return HD_E_INVALID;
+    if (hd::check_setup(coeffs, cfg)) return HD_E_INVALID;  // refused now, not when its context is made
     auto r = std::make_unique<Run>();
     const size_t n = (size_t)(mesh->M + 3) * (mesh->N + 3), T = mesh->total_time;
     r->mesh = *mesh;

@@ -142,7 +142,10 @@ int hd_finalize_coeffs(hd_coeffs *c);
 
 /* attribute_arrays_memory_alloc + supporting_arrays_alloc
  * (loader.cpp:224-375, main.cpp:216-233): allocate every device buffer of
- * one scenario and upload its inputs. */
+ * one scenario and upload its inputs.  HD_E_INVALID for M or N below 4, a
+ * missing required array or boundary series, fewer than 2 series samples,
+ * an unknown form or quirk bit, or dX, dY, dT not positive and finite;
+ * ops->M and ops->N are taken from the mesh. */
 int hd_create(const hd_mesh *mesh, const hd_coeffs *coeffs, const hd_options *ops,
               const hd_config *cfg, hd_ctx **out);
 

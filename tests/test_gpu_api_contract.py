@@ -123,6 +123,12 @@ def test_server_calls_with_bad_ids_sizes_and_states():
     cfg = engine.make_config()
     assert L.hd_server_submit(h, C.byref(tiny.mesh), C.byref(coeffs), C.byref(ops), C.byref(cfg),
                               C.byref(rid)) == INVALID
+    # so is a setup hd_create would refuse (here a zero time step)
+    good = engine.MeshHolder(sc)
+    zero_dt = engine.load_coeffs()
+    zero_dt.dT = 0.0
+    assert L.hd_server_submit(h, C.byref(good.mesh), C.byref(zero_dt), C.byref(ops), C.byref(cfg),
+                              C.byref(rid)) == INVALID
     a = srv.submit(sc, 40000.0)  # holds the one slot until cancelled
     b = srv.submit(S.open_box(20, 14, seed=3), 8.0)  # queued behind it
     out = np.zeros((sc.N + 3, sc.M + 3))
@@ -143,3 +149,43 @@ def test_server_calls_with_bad_ids_sizes_and_states():
     assert L.hd_server_release(h, a) == 0
     assert L.hd_server_poll(h, a, C.byref(st), C.byref(tt)) == INVALID  # released
     srv.close()
+
+
+def test_create_refuses_bad_meshes_and_setups():
+    print("[case] hd_create argument checks", flush=True)
+    L = engine.lib()
+    sc = S.open_box(20, 14)
+
+    def create(mesh=None, cfg=None, coeffs=None):
+        holder = abi.MeshHolder(sc)
+        c = engine.load_coeffs()
+        o = abi.make_options(sc, 8.0)
+        f = abi.make_config()
+        for obj, fn in ((holder.mesh, mesh), (f, cfg), (c, coeffs)):
+            if fn:
+                fn(obj)
+        h = C.c_void_p()
+        rc = L.hd_create(C.byref(holder.mesh), C.byref(c), C.byref(o), C.byref(f), C.byref(h))
+        if rc == 0:
+            L.hd_destroy(h)
+        return rc
+
+    assert create() == 0
+    bad = [
+        dict(mesh=lambda m: setattr(m, "M", 3)),
+        dict(mesh=lambda m: setattr(m, "N", -5)),
+        dict(mesh=lambda m: setattr(m, "h", None)),
+        dict(mesh=lambda m: setattr(m, "total_time", 1)),
+        dict(mesh=lambda m: m.bc.__setitem__(1, None)),
+        dict(cfg=lambda f: setattr(f, "form", 3)),
+        dict(cfg=lambda f: setattr(f, "quirks", 0x2)),
+        dict(coeffs=lambda c: setattr(c, "dT", 0.0)),
+        dict(coeffs=lambda c: setattr(c, "dX", -10.0)),
+        dict(coeffs=lambda c: setattr(c, "dY", float("nan"))),
+        dict(coeffs=lambda c: setattr(c, "dT", float("inf"))),
+    ]
+    for kw in bad:
+        assert create(**kw) == INVALID, kw
+    assert create(cfg=lambda f: setattr(f, "quirks", abi.HD_QUIRK_Q11_LITERAL if hasattr(abi, "HD_QUIRK_Q11_LITERAL")
+                                         else 1)) == 0
+    assert create(cfg=lambda f: setattr(f, "device", 99)) == -6  # HD_E_NODEV
