@@ -28,7 +28,7 @@ engine._lib = None
 FAST, PLAIN = abi.HD_FORM_FP64_FAST, abi.HD_FORM_FP64_PLAIN
 big = S.open_box(36, 24, seed=5)
 srv = engine.Server(hbm_budget=engine.footprint(big) + 1, max_active=8, steps_per_quantum=2, use_graph=True)
-first = srv.submit(big, 10.0, form=PLAIN)
+first = srv.submit(big, 20.0, form=PLAIN)  # still running when the clients submit
 errors = []
 
 def client(k):
