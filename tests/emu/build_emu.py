@@ -30,6 +30,14 @@ ASAN = ["-fsanitize=address", "-fno-omit-frame-pointer"]
 # and the API calls of client threads; the emulator announces its fibers)
 OUT_TSAN = os.path.join(ROOT, "tests", "_build", "emu_tsan")
 TSAN = ["-fsanitize=thread", "-fno-omit-frame-pointer"]
+# and with UndefinedBehaviorSanitizer, which here also covers the kernels
+# (compiled as C++ for the CPU): integer overflow in index arithmetic,
+# out-of-range shifts, misaligned or null accesses, invalid bool and enum
+# loads; the first finding aborts
+OUT_UBSAN = os.path.join(ROOT, "tests", "_build", "emu_ubsan")
+UBSAN = ["-fsanitize=undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer"]
+VARIANTS = {False: (OUT, []), True: (OUT_ASAN, ASAN), "address": (OUT_ASAN, ASAN), "thread": (OUT_TSAN, TSAN),
+            "undefined": (OUT_UBSAN, UBSAN)}
 LAUNCH = re.compile(r"([A-Za-z_][\w:]*(?:<[^<>;(){}]*>)?)\s*<<<(.*?)>>>\s*\(", re.S)
 
 
@@ -61,10 +69,9 @@ def _inputs():
 def build(verbose=False, sanitize=False, force=False):
     """Builds unless the outputs are newer than every input (force: always);
     concurrent callers (test workers) wait on a lock and build once."""
-    if sanitize not in (False, True, "address", "thread"):
+    if sanitize not in VARIANTS:
         raise ValueError(f"sanitize: {sanitize!r}")
-    tsan = sanitize == "thread"
-    out = OUT_TSAN if tsan else (OUT_ASAN if sanitize else OUT)
+    out = VARIANTS[sanitize][0]
     os.makedirs(out, exist_ok=True)
     lib = os.path.join(out, "libhydist_emu.so")
     drv = os.path.join(out, "compat_driver")
@@ -82,12 +89,9 @@ def _build(verbose=False, sanitize=False):
     """Builds the emulated library and the compat driver against it into
     tests/_build/emu (sanitize True or "address": with AddressSanitizer,
     into tests/_build/emu_asan; "thread": with ThreadSanitizer, into
-    tests/_build/emu_tsan).  Returns the library's path."""
-    if sanitize not in (False, True, "address", "thread"):
-        raise ValueError(f"sanitize: {sanitize!r}")
-    tsan = sanitize == "thread"
-    out = OUT_TSAN if tsan else (OUT_ASAN if sanitize else OUT)
-    extra = TSAN if tsan else (ASAN if sanitize else [])
+    tests/_build/emu_tsan; "undefined": with UndefinedBehaviorSanitizer,
+    into tests/_build/emu_ubsan).  Returns the library's path."""
+    out, extra = VARIANTS[sanitize]
     lib = os.path.join(out, "libhydist_emu.so")
     src_dir = os.path.join(out, "src")
     os.makedirs(src_dir, exist_ok=True)

@@ -9,6 +9,13 @@ EMU_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "em
 
 
 def use_emulator():
+    """HD_EMU_LIB names another emulated build (a sanitized one) to load
+    instead of tests/_build/emu's."""
+    other = os.environ.get("HD_EMU_LIB")
+    if other:
+        engine.LIB_PATH = other
+        engine._lib = None
+        return engine.lib()
     if not os.path.exists(EMU_LIB):
         import sys
         sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu"))
