@@ -1,12 +1,13 @@
-"""The library's host code under AddressSanitizer, on the CPU: the
-emulated build (tests/emu) compiled with -fsanitize=address runs the
-reference-surface driver (compat layer: input loading, contexts, the time
-loop, save_result) and the serving loop with a batched group, a context
-made on the helper thread and a cancelled run.  A heap or stack error in
-the host code ends the child with ASan's report.  (The emulated device
-buffers stay checked by their guard pages; the drop-in path's context
-lives for the process, as the reference's allocations do, so leak checks
-are off.)"""
+"""The library under AddressSanitizer, on the CPU: the emulated build
+(tests/emu) compiled with -fsanitize=address runs the reference-surface
+driver (compat layer: input loading, contexts, the time loop,
+save_result), the serving loop with a batched group, a context made on
+the helper thread and a cancelled run, and a cross-section of the GPU
+cases, whose kernels are C++ for the CPU here, so a per-lane array indexed
+out of its bounds is caught too.  A heap or stack error ends the child
+with ASan's report.  (The emulated device buffers stay checked by their
+guard pages; the drop-in path's context lives for the process, as the
+reference's allocations do, so leak checks are off.)"""
 import os
 import subprocess
 import sys
@@ -69,3 +70,18 @@ def test_serving_loop_under_asan(asan_build):
     r = subprocess.run([sys.executable, "-c", SERVER.format(root=ROOT, lib=lib)], capture_output=True, text=True,
                        timeout=600, env=env)
     assert r.returncode == 0 and "server ok" in r.stdout, r.stdout[-2000:] + r.stderr[-4000:]
+
+
+# kernels of every solver and form, the tridiagonal entry points and a
+# degenerate mesh
+CASES = " or ".join([
+    "gtsv", "noalloc", "(random_scenario and [0-)", "(thomas_per_system and [box])",
+    "(test_plain_fp64_bitwise and 12_0)", "test_fast_form_sediment",
+    "(test_every_line_solver_close_to_oracle and part4)", "(edge_mesh_fast and onecell)",
+])
+
+
+def test_gpu_cases_under_asan(asan_build):
+    import test_emu_suite as suite
+    lib, _, rt = asan_build
+    suite.run_cases(CASES, 15, HD_EMU_LIB=lib, LD_PRELOAD=rt, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
