@@ -26,7 +26,7 @@ SELECT = " or ".join([
     "gtsv", "noalloc", "out_of_order_and_bad", "bad_ids_sizes", "create_refuses", "restart_continues", "restore_rejects", "test_server", "two_runs", "footprint_is", "(random_scenario and ([0- or [1- or [2-))",
     "(thomas_per_system and ([box] or [onecell] or [flats]))", "plain_tuning", "fp64_forms", "in_the_fast_form", "fast_form_thomas",
     "(test_plain_fp64_bitwise and (12_0 or 12_1 or -20] or -25]))",
-    "half_steps", "sediment_and_bed_change", "lake_at_rest", "stage_entry_points", "q11_literal",
+    "half_steps", "sediment_and_bed_change", "lake_at_rest", "drying_threshold", "stage_entry_points", "q11_literal",
     "init_matches", "mass_balance_gpu", "output_callback",
     "(test_fast_form_close_to_oracle and 10_)", "test_fast_form_sediment",
     "(test_every_line_solver_close_to_oracle and part4)",

@@ -122,6 +122,25 @@ def test_lake_at_rest_gpu():
         assert not np.any(g.get(f)), f
 
 
+def test_drying_threshold_is_inclusive():
+    """Reset_states_vertical dries a wet cell when H_moi <= H_TINH
+    (support_funcs.cpp:142-188), equality included.  A lake at rest keeps
+    z exactly 0, so H_moi is exactly the cell-centre depth (4 m here);
+    with H_TINH set to that depth every interior wet cell sits exactly on
+    the threshold and must dry, as in the oracle."""
+    sc = S.closed_basin(30, 22, depth=4.0)
+    sc.coeff_overrides = dict(sc.coeff_overrides, H_TINH=4.0)
+    g, o = make_pair(sc)
+    wet0 = int(np.sum(g.get("KHOUOT") == 0))
+    g.step(2)
+    o.step(2)
+    kh = g.get("KHOUOT")
+    assert np.array_equal(kh, o.get("KHOUOT"))
+    assert wet0 > 0 and int(np.sum(kh == 0)) < wet0 // 2, (wet0, int(np.sum(kh == 0)))
+    for f in HYDRO:
+        assert bits_equal(g.get(f), o.get(f)), f
+
+
 def test_plain_fp64_one_hour_config1_bitwise():
     """Config 1 for one simulated hour (1800 steps): still bit for bit."""
     sc = S.config1()
