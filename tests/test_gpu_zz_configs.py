@@ -204,9 +204,9 @@ def test_c3_braided_sediment():
     e100["FS_rel"] = rel(at100["fp32"]["FS"], ref["FS"])
     e600 = {n: absdiff(end_fp32[n], end_plain[n]) for n in ("Z", "U", "V")}
     e600["FS_rel"] = rel(end_fp32["FS"], end_plain["FS"])
-    print(f"[c3] plain vs oracle @100: hydraulics bitwise, FS rel {fs_plain:.3e}; "
-          f"fp32 vs oracle @100 {e100}; fp32 vs plain @600 {e600}; max FS {np.nanmax(np.abs(end_plain['FS'])):.3e}",
-          flush=True)
+    print(f"[c3] plain vs oracle @{C3_ORACLE_STEPS}: hydraulics bitwise, FS rel {fs_plain:.3e}; "
+          f"fp32 vs oracle @{C3_ORACLE_STEPS} {e100}; fp32 vs plain @{C3_STEPS} {e600}; "
+          f"max FS {np.nanmax(np.abs(end_plain['FS'])):.3e}", flush=True)
     assert fs_plain <= FS_REL[PLAIN]
     assert np.array_equal(at100["fp32"]["KHOUOT"], ref["KHOUOT"])
     assert np.array_equal(end_fp32["KHOUOT"], end_plain["KHOUOT"])
